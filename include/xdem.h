@@ -1,0 +1,188 @@
+/*
+ * xdem.h -- C-ABI of libxdem_hip.so, the MI355X (gfx950) DEM step engine.
+ *
+ * The reference (xDEM) has no C-ABI: its plugin surface is C++ virtual
+ * dispatch on BaseWorld / BaseParticle / BaseContactModel.  Each entry point
+ * below replaces one group of those members; the citation names the member
+ * it stands in for.  Plain pointers and sizes only: no C++ or torch types.
+ *
+ * Conventions
+ *  - Every entry point returns 0 (XDEM_OK) or a negative XDEM_E_* code; the
+ *    text of the last failure is available from xdem_last_error().  The
+ *    reference prints and calls exit() where this library returns a code.
+ *  - Host arrays are caller-owned; the library copies them in and out.
+ *  - One host thread drives one engine (the reference's worlds are not
+ *    re-entrant either).  Device memory is owned by the engine.
+ *  - Particle ID == index inside its packing, as the reference requires
+ *    (src/BaseWorld.cpp:67-72).  Several independent packings ("worlds") may
+ *    share one engine; particles of packing k occupy the global index range
+ *    [packing_start[k], packing_start[k+1]).  Contact indices are global.
+ */
+#ifndef XDEM_H
+#define XDEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define XDEM_OK                  0
+#define XDEM_E_INVALID          (-1)  /* bad argument or call order */
+#define XDEM_E_CAPACITY         (-2)  /* candidate list >= 200: reference exit(-2),
+                                         src/PeriodicBoundaryWorld.cpp:75-78; or an
+                                         engine table limit (message says which) */
+#define XDEM_E_UNSUPPORTED_PAIR (-3)  /* candidate pair of unlike shapes: reference
+                                         exit(-1), src/EllipticParticle.cpp:30,
+                                         src/CircularParticle.cpp:38 */
+#define XDEM_E_ZERO_NORMAL      (-4)  /* normalize() of a zero vector: reference
+                                         exit(-1), src/MathVector.h:187-188 */
+#define XDEM_E_NONFINITE        (-5)  /* non-finite state detected by a check */
+#define XDEM_E_NEWTON_CAP       (-6)  /* quartic Newton hit its iteration cap; the
+                                         reference loop is uncapped and would hang,
+                                         src/EllipticParticle.cpp:613-626 */
+#define XDEM_E_HIP              (-7)  /* HIP runtime error */
+#define XDEM_E_GRID             (-8)  /* cell grid smaller than 3x3 per packing */
+
+/* world kinds (src/PeriodicBoundaryWorld.h, src/LeesEdwardsBoundaryWorld.h) */
+#define XDEM_WORLD_PERIODIC      0
+#define XDEM_WORLD_LEES_EDWARDS  1
+
+/* particle kinds (src/CircularParticle.h, src/EllipticParticle.h,
+   src/GluedParticle.h) */
+#define XDEM_KIND_CIRCLE   0
+#define XDEM_KIND_ELLIPSE  1
+#define XDEM_KIND_CLUMP    2
+
+/* per-particle SETTING bits, same values as src/BaseParticle.h:19-24 */
+#define XDEM_SKIP_TIME_INTEGRAL     1
+#define XDEM_SKIP_CONTACT_DETECTION 2
+#define XDEM_SKIP_MACRO_SUM         4
+
+/* precision modes */
+#define XDEM_PREC_FP64   0  /* all FP64, reference operation order */
+#define XDEM_PREC_MIXED  1  /* FP32 binning/culling/force sums, FP64 quartic,
+                               geometry and integration */
+#define XDEM_PREC_NN     2  /* FP64 path with the learned quartic-root guess */
+
+typedef struct xdem_engine xdem_engine;   /* opaque */
+
+/* One packing's world + contact model.  Replaces the constructor arguments
+ * of PeriodicBoundaryWorld / LeesEdwardsBoundaryWorld (src/BaseWorld.h:44-56),
+ * setWorldBoundary (src/BaseWorld.cpp:7-15), setShiftStrainRate
+ * (src/LeesEdwardsBoundaryWorld.cpp:485-489), BaseParticle::_GlobalDamping_
+ * (src/BaseParticle.cpp:9) and ContactModel_LinearDissipativeFrictioanl's
+ * constructor (src/ContactModel_LinearDissipativeFrictional.h:14-23). */
+typedef struct xdem_config {
+    int    world_kind;       /* XDEM_WORLD_* */
+    int    precision;        /* XDEM_PREC_*; must be equal for all packings */
+    double xmin, xmax, ymin, ymax;
+    double dt;
+    double kn, kt, restitution, mu;
+    double global_damping;   /* gamma of the integrator */
+    double shear_rate;       /* LE: gamma-dot (shift strain rate) */
+    double total_shift;      /* LE: initial boundary shift S */
+    double total_strain;     /* LE: initial accumulated strain */
+    double cell_size;        /* <= 0: 2.01 * max outer radius
+                                (src/BaseWorld.cpp:201-219) */
+    int    newton_cap;       /* <= 0: 200 Newton updates per pair */
+    int    strict_newton;    /* 1: a pair that reaches the cap fails the step
+                                with XDEM_E_NEWTON_CAP; 0: bisection rescue */
+} xdem_config;
+
+/* Create an engine holding num_packings independent worlds. */
+int xdem_create(xdem_engine** out, int num_packings, const xdem_config* cfgs);
+void xdem_destroy(xdem_engine* e);
+const char* xdem_last_error(const xdem_engine* e);
+
+/* Particle state.  Replaces constructing EllipticParticle / CircularParticle
+ * / GluedParticle objects plus calculateParticleProperties
+ * (src/EllipticParticle.cpp:7-14, src/CircularParticle.cpp:16-23) and
+ * handing their pointers to the world.  packing_start has num_packings+1
+ * entries (NULL allowed for one packing).  mass and inertia are the values
+ * the reference's calculateParticleProperties produces; clump_bx/by (may be
+ * NULL) are a clump sub-particle's branch vector from its clump centre
+ * (src/GluedParticle.cpp:71-72), unused for other particles. */
+int xdem_set_particles(xdem_engine* e, int n, const int* packing_start,
+                       const int* kind, const int* up_level, const int* flags,
+                       const double* outer_r, const double* aspect,
+                       const double* mass, const double* inertia,
+                       const double* x, const double* y, const double* theta,
+                       const double* vx, const double* vy, const double* omega,
+                       const double* clump_bx, const double* clump_by);
+
+/* Contact history restart: the persistent part of ContactInformation
+ * (src/ContactInformation.h:15-29) for contacts (i<j, history held by i as
+ * in src/BaseParticle.cpp:36).  Forces of restarted contacts are zero until
+ * the next step recomputes them. */
+int xdem_set_contacts(xdem_engine* e, int nc, const int* i, const int* j,
+                      const double* xi_x, const double* xi_y,
+                      const double* n_x, const double* n_y,
+                      const double* damping);
+
+/* Advance every packing by nsteps of takeTimeStep
+ * (src/LeesEdwardsBoundaryWorld.cpp:21-31, src/PeriodicBoundaryWorld.cpp:119-127).
+ * Asynchronous on the engine's stream; errors raised inside kernels are
+ * reported by the next xdem_sync. */
+int xdem_step(xdem_engine* e, int nsteps);
+int xdem_sync(xdem_engine* e);
+
+/* State read-back (BaseParticle getters, src/BaseParticle.h:109-127). */
+int xdem_get_particles(xdem_engine* e, double* x, double* y, double* theta,
+                       double* vx, double* vy, double* omega);
+
+/* Per-particle force and torque of the last step
+ * (BaseWorld::_totalForce/_totalTorque, src/BaseWorld.cpp:17-57) and the
+ * per-particle contact count (_numContactPerParticle). */
+int xdem_get_forces(xdem_engine* e, double* fx, double* fy, double* torque,
+                    int* num_contacts);
+
+/* Live contacts of the last step, ordered by (i, j) ascending.  *nc receives
+ * the count; arrays are filled up to cap entries.  Any output pointer may be
+ * NULL.  Mirrors ContactInformation fields (src/ContactInformation.h:15-29). */
+int xdem_get_contacts(xdem_engine* e, int cap, int* nc, int* i, int* j,
+                      double* fn_x, double* fn_y, double* fs_x, double* fs_y,
+                      double* point_x, double* point_y,
+                      double* xi_x, double* xi_y, double* n_x, double* n_y,
+                      double* damping, double* torque_i, double* torque_j);
+
+/* Stress of each packing, BaseWorld::updateTotalStress
+ * (src/BaseWorld.cpp:158-176): out[4k..4k+3] = xx, xy, yx, yy. */
+int xdem_get_stress(xdem_engine* e, double* out);
+
+/* LE scalars per packing: out[3k..3k+2] = total shift, shift velocity,
+ * total strain (src/LeesEdwardsBoundaryWorld.h:8-11). */
+int xdem_get_boundary(xdem_engine* e, double* out);
+
+/* Counters since creation: Newton iterations, bisection rescues of
+ * non-converging Newton runs, fallbacks to the plain root path (NN mode),
+ * candidate pairs visited, pairs that passed the bounding-circle test, steps. */
+int xdem_get_counters(xdem_engine* e, long long* newton_iters,
+                      long long* rescues, long long* fallbacks,
+                      long long* candidates, long long* coarse_pairs,
+                      long long* steps);
+
+/* Kernel timing: when enabled, HIP events bracket every launch of the
+ * contact kernel on the engine's stream; xdem_get_timing returns the summed
+ * milliseconds and the launch count since it was enabled. */
+int xdem_set_timing(xdem_engine* e, int enable);
+int xdem_get_timing(xdem_engine* e, double* contact_ms, long long* launches);
+
+/* Per-step algorithmic HBM bytes of the last step (particle state + contact
+ * history read and write) and the live contact count. */
+int xdem_get_traffic(xdem_engine* e, double* bytes_per_step, long long* contacts);
+
+/* Learned quartic-root guess (XDEM_PREC_NN).  Weights of the 8-64-64-1 MLP
+ * as FP32 host arrays (row-major W[out][in]); stored as BF16 on device. */
+int xdem_set_mlp(xdem_engine* e, const float* w1, const float* b1,
+                 const float* w2, const float* b2,
+                 const float* w3, const float* b3);
+
+/* Sample (features, converged root) pairs from the next step: up to cap
+ * rows of 8 features + the root + the plain Newton iteration count. */
+int xdem_log_roots(xdem_engine* e, int cap, int* count, float* features,
+                   double* roots, int* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XDEM_H */

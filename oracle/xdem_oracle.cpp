@@ -1,0 +1,872 @@
+// xdem_oracle.cpp -- CPU oracle for the DEM step loop.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, in plain FP64 C++, the
+// reference's step loop (Xiaoyu-JIANG/xDEM, src/) so that the HIP engine can
+// be checked against it.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it; nothing in the product links it.
+//
+// Faithfulness rules:
+//  * every floating-point expression keeps the reference's evaluation order
+//    (compile with -ffp-contract=off and no -march, like the reference's
+//    Release build: no FMA contraction);
+//  * transcendental functions are the system libm (std::sin/cos/atan2/sqrt),
+//    as in the reference;
+//  * history is held by the lower-ID particle and keyed by the other ID
+//    (src/BaseParticle.cpp:36); force sums run p1 ascending, partner
+//    ascending (the reference's single-thread order up to the map type).
+//
+// Pinning (what checks each function, beyond retyping):
+//  * contact geometry, history, forces, integration, LE bookkeeping: golden
+//    vectors produced by the real reference compiled from its sources
+//    (oracle/Makefile -> oracle/_ref), tests/test_oracle_golden.py;
+//  * ZPM distance: closed forms (circle limit, side-by-side 2b, end-to-end
+//    2a, T-junction a+b) and an independent Perram-Wertheim contact-function
+//    check, tests/test_oracle_math.py;
+//  * Newton root: companion-matrix roots (numpy), tests/test_oracle_math.py;
+//  * candidate sets: brute-force O(N^2) image search, tests/test_oracle_math.py;
+//  * contact model: binary head-on collision restitution = e,
+//    tests/test_oracle_math.py.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#define XDEM_OK                  0
+#define XDEM_E_INVALID          (-1)
+#define XDEM_E_CAPACITY         (-2)
+#define XDEM_E_UNSUPPORTED_PAIR (-3)
+#define XDEM_E_ZERO_NORMAL      (-4)
+#define XDEM_E_NEWTON_CAP       (-6)
+#define XDEM_E_GRID             (-8)
+
+namespace {
+
+enum { KIND_CIRCLE = 0, KIND_ELLIPSE = 1, KIND_CLUMP = 2 };
+enum { SKIP_TIME_INTEGRAL = 1, SKIP_CONTACT_DETECTION = 2, SKIP_MACRO_SUM = 4 };
+enum { WORLD_PERIODIC = 0, WORLD_LE = 1 };
+
+const int kMaxCandidates = 200;      // src/BaseWorld.h:11
+
+struct Contact {                     // src/ContactInformation.h:15-29
+    double nx = 0, ny = 0;           // _contactNormal
+    double px = 0, py = 0;           // _contactPoint
+    double bx = 0, by = 0;           // _branchVector (centre to centre)
+    double b1x = 0, b1y = 0;         // _branchVectorFromP1
+    double b2x = 0, b2y = 0;         // _branchVectorFromP2
+    double xix = 0, xiy = 0;         // _shearOverlap
+    double fnx = 0, fny = 0;         // _normalForce
+    double fsx = 0, fsy = 0;         // _shearForce
+    double fx = 0, fy = 0;           // _force (= _forceOnP1)
+    double t1 = 0, t2 = 0;           // _torqueOnP1/_torqueOnP2
+    double c = -1.0;                 // _dampingConstant
+};
+
+struct Counters { long long newton = 0, coarse = 0, rescues = 0; };
+
+struct Geometry {                    // src/ContactGeometry.h:5-16
+    double px, py, nx, ny, b1x, b1y, b2x, b2y, overlap;
+    double rvx, rvy, nrvx, nrvy, srvx, srvy;
+};
+
+struct World {
+    // configuration
+    int world_kind = WORLD_PERIODIC;
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0, xlen = 0, ylen = 0;
+    double dt = 0, kn = 0, kt = 0, e = 0, mu = 0, factor = 0, gdamp = 0;
+    double rate = 0, shift = 0, shift_vel = 0, strain = 0;
+    double cell = 0;
+    int newton_cap = 200, strict_newton = 0;
+    // particles (index == ID)
+    int n = 0;
+    std::vector<int> kind, up, flags;
+    std::vector<double> r, ar, m, im, in, iin;
+    std::vector<double> x, y, th, vx, vy, w;
+    std::vector<double> cbx, cby;            // clump branch vectors (subs)
+    std::vector<std::map<int, Contact>> hist;
+    // outputs of the last step
+    std::vector<double> fx, fy, tq;
+    std::vector<int> ncont;
+    // counters
+    long long newton_iters = 0, candidates = 0, coarse = 0, steps = 0, rescues = 0;
+    std::string err;
+};
+
+// ---------------------------------------------------------------------------
+// Newton iteration for the ZPM quartic, src/EllipticParticle.cpp:613-626,
+// started from sqrt(delta+1) (src/EllipticParticle.cpp:163).  The reference
+// loop is uncapped and, when the eigenvalue order of eigen() swaps the
+// transformed semi-axes (delta < 0), can cycle forever.  Here it stops after
+// `cap` updates; the caller then rescues with quartic_bisect.
+static double newton_quartic(double A, double B, double C, double D, double E,
+                             double init, int cap, long* iters) {
+    double A0 = 4.0 * A;
+    double B0 = 3.0 * B;
+    double C0 = 2.0 * C;
+    double r0 = init;
+    double r = 0.0;
+    const double tol = 1e-10;
+    long it = 0;
+    while (std::fabs(r0 - r) > tol) {
+        if (it >= cap) { *iters = -1; return r0; }
+        r = r0;
+        double p2 = r0 * r0;
+        double p3 = r0 * r0 * r0;
+        double p4 = r0 * r0 * r0 * r0;
+        double num = A * p4 + B * p3 + C * p2 + D * r0 + E;
+        double den = A0 * p3 + B0 * p2 + C0 * r0 + D;
+        r0 -= num / den;
+        ++it;
+    }
+    *iters = it;
+    return r0;
+}
+
+// Rescue root when Newton did not converge: 64 bisections of the bracket
+// [min(1, s), max(1, s)], s = sqrt(delta+1), which holds q because
+// sin^2(psi) = (q^2-1)/delta lies in [0, 1] (src/EllipticParticle.cpp:167,204).
+// Returns false when f has no sign change on the bracket.
+static bool quartic_bisect(double A, double B, double C, double D, double E,
+                           double s, double* root) {
+    double lo = s < 1.0 ? s : 1.0, hi = s < 1.0 ? 1.0 : s;
+    double flo = (((A * lo + B) * lo + C) * lo + D) * lo + E;
+    double fhi = (((A * hi + B) * hi + C) * hi + D) * hi + E;
+    if (flo == 0.0) { *root = lo; return true; }
+    if (fhi == 0.0) { *root = hi; return true; }
+    if ((flo > 0.0) == (fhi > 0.0) || !(flo == flo) || !(fhi == fhi)) return false;
+    for (int k = 0; k < 64; ++k) {
+        double m = 0.5 * (lo + hi);
+        double fm = (((A * m + B) * m + C) * m + D) * m + E;
+        if ((fm > 0.0) == (flo > 0.0)) { lo = m; flo = fm; }
+        else hi = m;
+    }
+    *root = 0.5 * (lo + hi);
+    return true;
+}
+
+// relative velocity at the contact point, src/BaseParticle.cpp:11-25.
+static void relative_velocity(const World& W, int i, int j, double ovx,
+                              double ovy, const double* vxb, Geometry& g) {
+    // _velocity + branchVectorFromP1.cross(_rotationVel); cross(w) = (-w*y, w*x)
+    double ax = vxb[i] + (-W.w[i] * g.b1y);
+    double ay = W.vy[i] + W.w[i] * g.b1x;
+    // other->getVel() + offsetVel + branchVectorFromP2.cross(other->getRotVel())
+    double bx = (vxb[j] + ovx) + (-W.w[j] * g.b2y);
+    double by = (W.vy[j] + ovy) + W.w[j] * g.b2x;
+    g.rvx = ax - bx;
+    g.rvy = ay - by;
+    double s = g.rvx * g.nx + g.rvy * g.ny;
+    g.nrvx = g.nx * s;
+    g.nrvy = g.ny * s;
+    g.srvx = g.rvx - g.nrvx;
+    g.srvy = g.rvy - g.nrvy;
+}
+
+// circle-circle contact, src/CircularParticle.cpp:67-92.
+static bool contact_circles(const World& W, int i, int j, double ox, double oy,
+                            double ovx, double ovy, const double* vxb,
+                            Geometry& g) {
+    double pjx = W.x[j] + ox, pjy = W.y[j] + oy;
+    double bvx = pjx - W.x[i], bvy = pjy - W.y[i];
+    double distance = std::sqrt(bvx * bvx + bvy * bvy);
+    double sum_r = W.r[i] + W.r[j];
+    g.overlap = sum_r - distance;
+    if (g.overlap < 0.0) return false;
+    double ls = bvx * bvx + bvy * bvy;          // getUnitVector, MathVector.h:169-178
+    if (ls != 0.0) { double s = std::sqrt(ls); g.nx = bvx / s; g.ny = bvy / s; }
+    else { g.nx = 0.0; g.ny = 0.0; }
+    double p1x = W.x[i] + g.nx * W.r[i], p1y = W.y[i] + g.ny * W.r[i];
+    double p2x = pjx - g.nx * W.r[j], p2y = pjy - g.ny * W.r[j];
+    g.px = (p1x + p2x) / 2.0;
+    g.py = (p1y + p2y) / 2.0;
+    g.b1x = g.px - W.x[i]; g.b1y = g.py - W.y[i];
+    g.b2x = g.px - pjx;    g.b2y = g.py - pjy;
+    relative_velocity(W, i, j, ovx, ovy, vxb, g);
+    return true;
+}
+
+// abs((int)v) as x86-64 evaluates it: truncation toward zero, and the
+// "integer indefinite" INT_MIN for NaN or out-of-range values (cvttsd2si),
+// whose abs() stays INT_MIN.
+static inline long long int_abs_of(double v) {
+    const double lim = 2147483648.0;
+    if (!(v > -lim - 1.0 && v < lim)) return -2147483648LL;
+    long long k = (long long)v;
+    return k < 0 ? -k : k;
+}
+
+// 2x2 product as written in src/MathMatrix2d.h:85-91 (its YY term takes
+// o.yx; harmless for the symmetric operands used here, kept as written).
+struct M2 { double xx, xy, yx, yy; };
+static inline M2 mul(const M2& a, const M2& o) {
+    return M2{a.xx * o.xx + a.xy * o.yx, a.xx * o.xy + a.xy * o.yy,
+              a.yx * o.xx + a.yy * o.yx, a.yx * o.yx + a.yy * o.yy};
+}
+
+// ZPM ellipse-ellipse contact, src/EllipticParticle.cpp:16-32 and 34-285.
+// Returns 0 no contact, 1 contact, or a negative error code.
+static int contact_ellipses(const World& W, int i, int j, double ox, double oy,
+                            double ovx, double ovy, const double* vxb,
+                            Geometry& g, Counters& K) {
+    // coarse bounding-circle test, src/BaseParticle.h:131-132
+    {
+        double qx = W.x[i] - (W.x[j] + ox), qy = W.y[i] - (W.y[j] + oy);
+        if (!((W.r[i] + W.r[j]) > std::sqrt(qx * qx + qy * qy))) return 0;
+    }
+    K.coarse++;
+    const double a1 = W.r[i], a2 = W.r[j];
+    const double b1 = a1 / W.ar[i], b2 = a2 / W.ar[j];
+    const double z1 = W.th[i], z2 = W.th[j];
+    const double dx = (W.x[j] + ox) - W.x[i], dy = (W.y[j] + oy) - W.y[i];
+    const double dlen = std::sqrt(dx * dx + dy * dy);
+    const double dhx = dx / dlen, dhy = dy / dlen;
+    const double k1x = std::cos(z1), k1y = std::sin(z1);
+    const double k2x = std::cos(z2), k2y = std::sin(z2);
+    const M2 k1k1{k1x * k1x, k1x * k1y, k1y * k1x, k1y * k1y};
+    const M2 k2k2{k2x * k2x, k2x * k2y, k2y * k2x, k2y * k2y};
+    const double e1 = std::sqrt(1.0 - (b1 * b1) / (a1 * a1));
+    const double e2 = std::sqrt(1.0 - (b2 * b2) / (a2 * a2));
+    // T = (I + k1k1*(b1/a1-1)) / b1, eq(4)
+    const double s1 = b1 / a1 - 1.0;
+    const M2 T{(1.0 + k1k1.xx * s1) / b1, (0.0 + k1k1.xy * s1) / b1,
+               (0.0 + k1k1.yx * s1) / b1, (1.0 + k1k1.yy * s1) / b1};
+    const double eta = a1 / b1 - 1.0;
+    const double dphx = T.xx * dhx + T.xy * dhy, dphy = T.yx * dhx + T.yy * dhy;
+    const double dp0x = T.xx * dx + T.xy * dy, dp0y = T.yx * dx + T.yy * dy;
+    // A' = (I+k1k1*eta)*(I-k2k2*e2^2)*(I+k1k1*eta)*b1^2/b2^2, eq(8)
+    const M2 M1{1.0 + k1k1.xx * eta, 0.0 + k1k1.xy * eta,
+                0.0 + k1k1.yx * eta, 1.0 + k1k1.yy * eta};
+    const double e2s = e2 * e2;
+    const M2 Mb{1.0 - k2k2.xx * e2s, 0.0 - k2k2.xy * e2s,
+                0.0 - k2k2.yx * e2s, 1.0 - k2k2.yy * e2s};
+    M2 Ap = mul(mul(M1, Mb), M1);
+    const double b1s = b1 * b1, b2s = b2 * b2;
+    Ap = M2{Ap.xx * b1s, Ap.xy * b1s, Ap.yx * b1s, Ap.yy * b1s};
+    Ap = M2{Ap.xx / b2s, Ap.xy / b2s, Ap.yx / b2s, Ap.yy / b2s};
+    // eigen, src/MathMatrix2d.h:112-135
+    double lx, ly, kpx, kpy, kmx, kmy;
+    {
+        double tr = Ap.xx + Ap.yy;
+        double det = Ap.xx * Ap.yy - Ap.xy * Ap.yx;
+        double disc = tr * tr - 4.0 * det;
+        if (disc < 0) disc = 0.0;
+        disc = std::sqrt(disc);
+        double ev1 = (tr + disc) / 2.0, ev2 = (tr - disc) / 2.0;
+        // MathMatrix2d.h is parsed before any header declares abs(double),
+        // so its unqualified abs() is C's int abs(int): the eigenvalues are
+        // compared after truncation to int (src/MathMatrix2d.h:122).
+        if (int_abs_of(ev1) > int_abs_of(ev2)) { lx = ev1; ly = ev2; }
+        else { lx = ev2; ly = ev1; }
+        kpx = 1.0; kpy = (lx - Ap.xx) / Ap.xy;
+        double ls = kpx * kpx + kpy * kpy;
+        if (ls != 0.0) { double s = std::sqrt(ls); kpx /= s; kpy /= s; }
+        else return XDEM_E_ZERO_NORMAL;
+        kmx = 1.0; kmy = (ly - Ap.xx) / Ap.xy;
+        ls = kmx * kmx + kmy * kmy;
+        if (ls != 0.0) { double s = std::sqrt(ls); kmx /= s; kmy /= s; }
+        else return XDEM_E_ZERO_NORMAL;
+    }
+    const double a2p = 1.0 / std::sqrt(ly);
+    const double b2p = 1.0 / std::sqrt(lx);
+    if (std::sqrt(dp0x * dp0x + dp0y * dp0y) > 1.0 + a2p) return 0;
+    if (std::fabs(dp0x * kpx + dp0y * kpy) > 1.0 + b2p) return 0;
+    if (kmx * dphx + kmy * dphy < 0) { kmx = -kmx; kmy = -kmy; }
+    if (kpx * dphx + kpy * dphy < 0) { kpx = -kpx; kpy = -kpy; }
+    const double delta = (a2p * a2p) / (b2p * b2p) - 1;
+    const double tanphi = (kmx * dphx + kmy * dphy) / (kpx * dphx + kpy * dphy);
+    const double t2 = tanphi * tanphi;
+    // quartic coefficients, eq(26)
+    const double A = -(1.0 + t2) / (b2p * b2p);
+    const double B = -2.0 * (1.0 + t2 + delta) / b2p;
+    const double C = -t2 - (1.0 + delta) * (1.0 + delta) +
+                     (1.0 + (1.0 + delta) * t2) / (b2p * b2p);
+    const double D = 2.0 * (1.0 + t2) * (1.0 + delta) / b2p;
+    const double E = (1.0 + t2 + delta) * (1.0 + delta);
+    long iters = 0;
+    const double qinit = std::sqrt(delta + 1.0);
+    double q = newton_quartic(A, B, C, D, E, qinit, W.newton_cap, &iters);
+    if (iters < 0) {
+        if (W.strict_newton || !quartic_bisect(A, B, C, D, E, qinit, &q))
+            return XDEM_E_NEWTON_CAP;
+        K.rescues++;
+        iters = W.newton_cap;
+    }
+    K.newton += iters;
+    // eq(33)
+    double temp = (q * q - 1.0) / delta;
+    double dprime;
+    if (temp < 0) temp = 0.0;
+    if (delta == 0 || !std::isfinite(temp)) {
+        dprime = 1.0 + a2p;
+    } else {
+        double u = (1.0 + b2p * (1.0 + delta) / q);
+        double v = (1.0 + b2p / q);
+        dprime = std::sqrt(temp * (u * u) + (1.0 - temp) * (v * v));
+    }
+    // eq(36)
+    const double kd = k1x * dhx + k1y * dhy;
+    const double e1s = e1 * e1;
+    const double dist = dprime / std::sqrt(1.0 - (kd * kd) * e1s) * b1;
+    const double overlap0 = dist - std::sqrt(dx * dx + dy * dy);
+    if (overlap0 < 0.0) return 0;
+    // tangent point in the circle frame
+    double wx, wy;
+    if (delta == 0) {
+        wx = dhx; wy = dhy;
+    } else {
+        double sp2 = (q * q - 1.0) / delta;
+        if (sp2 < 0) sp2 = 0.0;
+        double cp2 = 1.0 - sp2;
+        if (cp2 < 0) cp2 = 0.0;
+        double sx = std::sqrt(sp2), sy = std::sqrt(cp2);
+        wx = kmx * sx + kpx * sy;          // K'^-1 = [km kp] as columns
+        wy = kmy * sx + kpy * sy;
+    }
+    // T.inv() = (yy,-xy,-yx,xx)/det, src/MathMatrix2d.h:137-140
+    const double tdet = T.xx * T.yy - T.xy * T.yx;
+    const M2 Ti{T.yy / tdet, -T.xy / tdet, -T.yx / tdet, T.xx / tdet};
+    const double tpx = Ti.xx * wx + Ti.xy * wy, tpy = Ti.yx * wx + Ti.yy * wy;
+    const double dl2 = std::sqrt(dx * dx + dy * dy);
+    const double cpx = tpx * dl2 / dist, cpy = tpy * dl2 / dist;
+    // A1 = (I - k1k1*e1^2) / b1^2
+    const M2 A1{(1.0 - k1k1.xx * e1s) / b1s, (0.0 - k1k1.xy * e1s) / b1s,
+                (0.0 - k1k1.yx * e1s) / b1s, (1.0 - k1k1.yy * e1s) / b1s};
+    double nx = A1.xx * tpx + A1.xy * tpy, ny = A1.yx * tpx + A1.yy * tpy;
+    {
+        double ls = nx * nx + ny * ny;
+        if (ls != 0.0) { double s = std::sqrt(ls); nx /= s; ny /= s; }
+        else return XDEM_E_ZERO_NORMAL;
+    }
+    double ovx0 = dhx * overlap0, ovy0 = dhy * overlap0;
+    double od = ovx0 * nx + ovy0 * ny;
+    ovx0 = nx * od; ovy0 = ny * od;
+    g.overlap = std::sqrt(ovx0 * ovx0 + ovy0 * ovy0);
+    g.nx = nx; g.ny = ny;
+    g.px = cpx + W.x[i]; g.py = cpy + W.y[i];
+    g.b1x = g.px - W.x[i]; g.b1y = g.py - W.y[i];
+    g.b2x = g.px - (W.x[j] + ox); g.b2y = g.py - (W.y[j] + oy);
+    relative_velocity(W, i, j, ovx, ovy, vxb, g);
+    return 1;
+}
+
+// src/ContactModel_LinearDissipativeFrictional.cpp:73-82
+static double damping_constant(const World& W, int i, int j) {
+    bool si = W.flags[i] & SKIP_TIME_INTEGRAL, sj = W.flags[j] & SKIP_TIME_INTEGRAL;
+    if (!si && sj) return std::sqrt(W.m[i]) * W.factor;
+    if (si && !sj) return std::sqrt(W.m[j]) * W.factor;
+    double m0 = (W.m[i] * W.m[j]) / (W.m[i] + W.m[j]);
+    return std::sqrt(m0) * W.factor;
+}
+
+// BaseParticle::calculateContactForceAndTorque, src/BaseParticle.cpp:27-98,
+// with the contact model of src/ContactModel_LinearDissipativeFrictional.cpp:3-46
+// and ContactInformation::calculateTotalForceAndTorque,
+// src/ContactInformation.h:62-68.
+static int contact_force(World& W, int i, int j, double ox, double oy,
+                         double ovx, double ovy, const double* vxb, Counters& K) {
+    Geometry g;
+    int hit;
+    int ki = W.kind[i], kj = W.kind[j];
+    if (ki == KIND_CLUMP) {
+        hit = 0;                                  // src/GluedParticle.cpp:82-89
+    } else if (ki == KIND_CIRCLE) {
+        if (kj != KIND_CIRCLE) return XDEM_E_UNSUPPORTED_PAIR;
+        K.coarse++;
+        hit = contact_circles(W, i, j, ox, oy, ovx, ovy, vxb, g) ? 1 : 0;
+    } else {
+        if (kj != KIND_ELLIPSE) return XDEM_E_UNSUPPORTED_PAIR;
+        hit = contact_ellipses(W, i, j, ox, oy, ovx, ovy, vxb, g, K);
+        if (hit < 0) return hit;
+    }
+    auto& h = W.hist[i];
+    if (!hit) { h.erase(j); return 0; }
+    auto it = h.find(j);
+    if (it == h.end()) {
+        Contact c;
+        c.nx = g.nx; c.ny = g.ny; c.px = g.px; c.py = g.py;
+        c.b1x = g.b1x; c.b1y = g.b1y; c.b2x = g.b2x; c.b2y = g.b2y;
+        double vn = g.rvx * g.nx + g.rvy * g.ny;
+        if (!std::isnormal(vn)) {
+            c.xix = g.srvx * W.dt; c.xiy = g.srvy * W.dt;
+        } else {
+            double t = g.overlap / vn;
+            double s = (t < W.dt) ? t : W.dt;    // std::min(dt, t)
+            c.xix = g.srvx * s; c.xiy = g.srvy * s;
+        }
+        c.c = damping_constant(W, i, j);
+        it = h.emplace(j, c).first;
+    } else {
+        Contact& c = it->second;
+        double dot = c.nx * g.nx + c.ny * g.ny;
+        double det = c.nx * g.ny - c.ny * g.nx;
+        double ang = std::atan2(det, dot);
+        double ca = std::cos(ang), sa = std::sin(ang);
+        double xx = ca * c.xix + (-sa) * c.xiy;
+        double yy = sa * c.xix + ca * c.xiy;
+        c.xix = xx + g.srvx * W.dt;
+        c.xiy = yy + g.srvy * W.dt;
+        c.nx = g.nx; c.ny = g.ny; c.px = g.px; c.py = g.py;
+        c.b1x = g.b1x; c.b1y = g.b1y; c.b2x = g.b2x; c.b2y = g.b2y;
+    }
+    Contact& c = it->second;
+    c.bx = W.x[i] - (W.x[j] + ox);
+    c.by = W.y[i] - (W.y[j] + oy);
+    // normal force: -n*kn*overlap + (-nrv*c)
+    c.fnx = (-g.nx) * W.kn * g.overlap + (-g.nrvx) * c.c;
+    c.fny = (-g.ny) * W.kn * g.overlap + (-g.nrvy) * c.c;
+    // shear force with Coulomb cap
+    double fmax = std::sqrt(c.fnx * c.fnx + c.fny * c.fny) * W.mu;
+    double ex = (-c.xix) * W.kt, ey = (-c.xiy) * W.kt;
+    double sx = ex + (-g.srvx) * c.c, sy = ey + (-g.srvy) * c.c;
+    if (std::sqrt(sx * sx + sy * sy) > fmax) {
+        double ls = sx * sx + sy * sy;
+        double ux = 0.0, uy = 0.0;
+        if (ls != 0.0) { double s = std::sqrt(ls); ux = sx / s; uy = sy / s; }
+        sx = ux * fmax; sy = uy * fmax;
+        if (std::sqrt(ex * ex + ey * ey) > fmax) {
+            c.xix = (-sx) / W.kt; c.xiy = (-sy) / W.kt;
+        } else {
+            c.xix = (-ex) / W.kt; c.xiy = (-ey) / W.kt;
+        }
+    }
+    c.fsx = sx; c.fsy = sy;
+    c.fx = c.fnx + c.fsx; c.fy = c.fny + c.fsy;
+    c.t1 = c.b1x * c.fy - c.b1y * c.fx;
+    c.t2 = c.b2x * (-c.fy) - c.b2y * (-c.fx);
+    return 1;
+}
+
+// one takeTimeStep of either world.
+static int step_once(World& W) {
+    const int n = W.n;
+    const bool le = W.world_kind == WORLD_LE;
+    // modifyParticlePosition: LE src/LeesEdwardsBoundaryWorld.cpp:304-333,
+    // periodic src/PeriodicBoundaryWorld.cpp:129-142
+    for (int i = 0; i < n; ++i) {
+        if (le) {
+            if (W.y[i] < W.ymin) { W.y[i] += W.ylen; W.x[i] += W.shift; }
+            else if (W.y[i] > W.ymax) { W.y[i] += -W.ylen; W.x[i] += -W.shift; }
+        } else {
+            if (W.y[i] < W.ymin) W.y[i] += W.ylen;
+            else if (W.y[i] > W.ymax) W.y[i] += -W.ylen;
+        }
+        if (W.x[i] < W.xmin) W.x[i] += W.xlen;
+        else if (W.x[i] > W.xmax) W.x[i] += -W.xlen;
+    }
+    // applyBackgroudVelocity, src/LeesEdwardsBoundaryWorld.cpp:508-518
+    std::vector<double> vxb(W.vx);
+    if (le) for (int i = 0; i < n; ++i) vxb[i] = W.vx[i] + W.rate * W.y[i];
+    // findPossibleContacts: grid, src/PeriodicBoundaryWorld.cpp:15-22
+    W.xlen = W.xmax - W.xmin;
+    W.ylen = W.ymax - W.ymin;
+    const int nx = (int)std::floor(W.xlen / W.cell);
+    const int ny = (int)std::floor(W.ylen / W.cell);
+    if (nx < 3 || ny < 3) { W.err = "cell grid smaller than 3x3"; return XDEM_E_GRID; }
+    const double lx = W.xlen / nx, ly = W.ylen / ny;
+    std::vector<int> cx(n), cy(n), ghost(n, -1);
+    // bins rows: 0 = ghost below (jy == -1), 1..ny = grid rows, ny+1 = ghost
+    // above (jy == ny), src/LeesEdwardsBoundaryWorld.cpp:53-124
+    std::vector<std::vector<int>> bins((size_t)(ny + 2) * nx);
+    for (int i = 0; i < n; ++i) {
+        int ix = (int)std::floor((W.x[i] - W.xmin) / lx);
+        int iy = (int)std::floor((W.y[i] - W.ymin) / ly);
+        ix = std::min(std::max(ix, 0), nx - 1);   // x == xmax: see DESIGN
+        iy = std::min(std::max(iy, 0), ny - 1);
+        cx[i] = ix; cy[i] = iy;
+        bins[(size_t)(iy + 1) * nx + ix].push_back(i);
+        double sh = le ? W.shift : 0.0;
+        if (iy == 0 || iy == ny - 1) {
+            double t = (iy == 0) ? W.x[i] + sh : W.x[i] - sh;
+            if (le) {
+                if (t < W.xmin) t += W.xlen;
+                else if (t > W.xmax) t -= W.xlen;
+            }
+            int gx = le ? (int)std::floor((t - W.xmin) / lx) : ix;
+            gx = std::min(std::max(gx, 0), nx - 1);
+            int row = (iy == 0) ? ny + 1 : 0;      // bottom row feeds ghost above
+            bins[(size_t)row * nx + gx].push_back(i);
+            ghost[i] = gx;
+        }
+    }
+    // updateContacts: candidates per i, src/PeriodicBoundaryWorld.cpp:47-106,
+    // src/LeesEdwardsBoundaryWorld.cpp:128-282
+    std::vector<int> status(n, 0);
+    std::vector<int> errj(n, -1);
+    const double hx = W.xlen / 2.0, hy = W.ylen / 2.0;
+    long long cand_total = 0, newton_total = 0, coarse_total = 0, rescue_total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : cand_total, newton_total, coarse_total, rescue_total)
+    for (int i = 0; i < n; ++i) {
+        Counters K;
+        if (!le && (W.flags[i] & SKIP_CONTACT_DETECTION)) continue;
+        std::vector<int> lists[3];   // 0 normal, 1 top (ghost above), 2 bottom
+        for (int jx0 = -1; jx0 <= 1; ++jx0) {
+            int jx = cx[i] + jx0;
+            if (jx == -1) jx = nx - 1;
+            else if (jx == nx) jx = 0;
+            for (int jy0 = -1; jy0 <= 1; ++jy0) {
+                int row = cy[i] + jy0 + 1;
+                int which = 0;
+                if (le && row == 0) which = 2;
+                else if (le && row == ny + 1) which = 1;
+                const auto& b = bins[(size_t)row * nx + jx];
+                for (int j : b) {
+                    if (!le && (W.flags[j] & SKIP_CONTACT_DETECTION)) continue;
+                    if (i >= j) continue;
+                    if (W.up[i] == W.up[j]) continue;
+                    lists[which].push_back(j);
+                }
+            }
+        }
+        if ((int)lists[0].size() >= kMaxCandidates ||
+            (int)lists[1].size() >= kMaxCandidates ||
+            (int)lists[2].size() >= kMaxCandidates) {
+            status[i] = XDEM_E_CAPACITY;
+            continue;
+        }
+        cand_total += lists[0].size() + lists[1].size() + lists[2].size();
+        for (int j : lists[0]) {
+            double ox = 0.0, oy = 0.0;
+            if (W.x[j] - W.x[i] > hx) ox = -W.xlen;
+            else if (W.x[j] - W.x[i] < -hx) ox = W.xlen;
+            if (W.y[j] - W.y[i] > hy) oy = -W.ylen;
+            else if (W.y[j] - W.y[i] < -hy) oy = W.ylen;
+            int rc = contact_force(W, i, j, ox, oy, 0.0, 0.0, vxb.data(), K);
+            if (rc < 0) { status[i] = rc; errj[i] = j; break; }
+        }
+        if (status[i] < 0) continue;
+        for (int t = 1; t <= 2; ++t) {
+            for (int j : lists[t]) {
+                double ox = (t == 1) ? W.shift : -W.shift;
+                double oy = (t == 1) ? W.ylen : -W.ylen;
+                double tx = ox + W.x[j];
+                if (tx < W.xmin) { ox += W.xlen; tx += W.xlen; }
+                else if (tx > W.xmax) { ox += -W.xlen; tx -= W.xlen; }
+                if (tx - W.x[i] > hx) ox += -W.xlen;
+                else if (tx - W.x[i] < -hx) ox += W.xlen;
+                double ovx = (t == 1) ? W.shift_vel : -W.shift_vel;
+                int rc = contact_force(W, i, j, ox, oy, ovx, 0.0, vxb.data(), K);
+                if (rc < 0) { status[i] = rc; errj[i] = j; break; }
+            }
+            if (status[i] < 0) break;
+        }
+        newton_total += K.newton;
+        coarse_total += K.coarse;
+        rescue_total += K.rescues;
+    }
+    W.candidates += cand_total;
+    W.newton_iters += newton_total;
+    W.coarse += coarse_total;
+    W.rescues += rescue_total;
+    for (int i = 0; i < n; ++i) {
+        if (status[i] < 0) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "step failed at particle %d (partner %d), code %d",
+                     i, errj[i], status[i]);
+            W.err = buf;
+            return status[i];
+        }
+    }
+    // removeBackgroudVelocity, src/LeesEdwardsBoundaryWorld.cpp:520-530
+    if (le) for (int i = 0; i < n; ++i) W.vx[i] = vxb[i] - W.rate * W.y[i];
+    // collectForceAndTorque, src/BaseWorld.cpp:17-57
+    std::fill(W.fx.begin(), W.fx.end(), 0.0);
+    std::fill(W.fy.begin(), W.fy.end(), 0.0);
+    std::fill(W.tq.begin(), W.tq.end(), 0.0);
+    std::fill(W.ncont.begin(), W.ncont.end(), 0);
+    for (int p1 = 0; p1 < n; ++p1) {
+        for (auto& kv : W.hist[p1]) {
+            int p2 = kv.first;
+            const Contact& c = kv.second;
+            W.fx[p1] += c.fx; W.fy[p1] += c.fy;
+            W.fx[p2] += -c.fx; W.fy[p2] += -c.fy;
+            W.tq[p1] += c.t1; W.tq[p2] += c.t2;
+            W.ncont[p1]++; W.ncont[p2]++;
+        }
+    }
+    // takeTimeIntegral, src/BaseWorld.cpp:59-77 -> src/BaseParticle.cpp:107-118,
+    // clumps src/GluedParticle.cpp:91-125
+    const double g = W.gdamp, dt = W.dt;
+    for (int i = 0; i < n; ++i) {
+        if (W.flags[i] & SKIP_TIME_INTEGRAL) continue;
+        if (W.kind[i] != KIND_CLUMP) {
+            W.vx[i] = (W.vx[i] * (1.0 - g * dt) + (W.fx[i] * W.im[i]) * dt) / (1.0 + g * dt);
+            W.vy[i] = (W.vy[i] * (1.0 - g * dt) + (W.fy[i] * W.im[i]) * dt) / (1.0 + g * dt);
+            W.x[i] += W.vx[i] * dt;
+            W.y[i] += W.vy[i] * dt;
+            W.w[i] = (W.w[i] * (1.0 - g * dt) + W.tq[i] * W.iin[i] * dt) / (1.0 + g * dt);
+            W.th[i] += W.w[i] * dt;
+        } else {
+            double tfx = 0.0, tfy = 0.0, tt = 0.0;
+            int s = i + 1;
+            for (; s < n && W.up[s] == i && s != i; ++s) {
+                tfx += W.fx[s]; tfy += W.fy[s];
+                tt += W.tq[s];
+                tt += W.cbx[s] * W.fy[s] - W.cby[s] * W.fx[s];
+            }
+            W.vx[i] = (W.vx[i] * (1.0 - g * dt) + tfx * W.im[i] * dt) / (1.0 + g * dt);
+            W.vy[i] = (W.vy[i] * (1.0 - g * dt) + tfy * W.im[i] * dt) / (1.0 + g * dt);
+            W.x[i] += W.vx[i] * dt;
+            W.y[i] += W.vy[i] * dt;
+            W.w[i] = (W.w[i] * (1.0 - g * dt) + tt * W.iin[i] * dt) / (1.0 + g * dt);
+            W.th[i] += W.w[i] * dt;
+            double ang = W.w[i] * dt;
+            double ca = std::cos(ang), sa = std::sin(ang);
+            for (int k = i + 1; k < s; ++k) {
+                double bx = ca * W.cbx[k] + (-sa) * W.cby[k];
+                double by = sa * W.cbx[k] + ca * W.cby[k];
+                W.cbx[k] = bx; W.cby[k] = by;
+                W.vx[k] = W.vx[i] + (-W.w[i] * by);
+                W.vy[k] = W.vy[i] + W.w[i] * bx;
+                W.x[k] = W.x[i] + bx;
+                W.y[k] = W.y[i] + by;
+                W.w[k] = W.w[i];
+                W.th[k] += ang;
+            }
+        }
+    }
+    // updateLeesEdwardBoundary, src/LeesEdwardsBoundaryWorld.cpp:285-296
+    if (le) {
+        W.shift_vel = W.rate * W.ylen;
+        W.strain += W.rate * W.dt;
+        W.shift += W.shift_vel * W.dt;
+        if (W.shift > W.xmax) W.shift -= W.xlen;
+        else if (W.shift < W.xmin) W.shift += W.xlen;
+    }
+    W.steps++;
+    return XDEM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API (mirrors include/xdem.h for one packing)
+extern "C" {
+
+struct oracle_config {
+    int world_kind; int precision;
+    double xmin, xmax, ymin, ymax, dt, kn, kt, restitution, mu;
+    double global_damping, shear_rate, total_shift, total_strain, cell_size;
+    int newton_cap, strict_newton;
+};
+
+void* oracle_create(const oracle_config* c) {
+    World* W = new World();
+    W->world_kind = c->world_kind;
+    // setWorldBoundary, src/BaseWorld.cpp:7-15
+    W->xmin = c->xmin; W->xmax = c->xmax; W->ymin = c->ymin; W->ymax = c->ymax;
+    W->xlen = W->xmax - W->xmin; W->ylen = W->ymax - W->ymin;
+    W->dt = c->dt; W->kn = c->kn; W->kt = c->kt; W->e = c->restitution; W->mu = c->mu;
+    // src/ContactModel_LinearDissipativeFrictional.h:21-23
+    double le = std::log(W->e);
+    W->factor = 2.0 * std::sqrt(W->kn) * std::fabs(le) / std::sqrt(M_PI * M_PI + le * le);
+    W->gdamp = c->global_damping;
+    W->rate = c->shear_rate;
+    // setShiftStrainRate, src/LeesEdwardsBoundaryWorld.cpp:485-489
+    W->shift_vel = W->rate * W->ylen;
+    W->shift = c->total_shift;
+    W->strain = c->total_strain;
+    W->cell = c->cell_size;
+    W->newton_cap = c->newton_cap > 0 ? c->newton_cap : 200;
+    W->strict_newton = c->strict_newton;
+    return W;
+}
+
+void oracle_destroy(void* h) { delete (World*)h; }
+
+const char* oracle_last_error(void* h) { return ((World*)h)->err.c_str(); }
+
+int oracle_set_particles(void* h, int n, const int* kind, const int* up,
+                         const int* flags, const double* r, const double* ar,
+                         const double* m, const double* inertia,
+                         const double* x, const double* y, const double* th,
+                         const double* vx, const double* vy, const double* w,
+                         const double* cbx, const double* cby) {
+    World& W = *(World*)h;
+    W.n = n;
+    W.kind.assign(kind, kind + n); W.up.assign(up, up + n); W.flags.assign(flags, flags + n);
+    W.r.assign(r, r + n); W.ar.assign(ar, ar + n);
+    W.m.assign(m, m + n); W.in.assign(inertia, inertia + n);
+    W.im.resize(n); W.iin.resize(n);
+    for (int i = 0; i < n; ++i) { W.im[i] = 1.0 / W.m[i]; W.iin[i] = 1.0 / W.in[i]; }
+    W.x.assign(x, x + n); W.y.assign(y, y + n); W.th.assign(th, th + n);
+    W.vx.assign(vx, vx + n); W.vy.assign(vy, vy + n); W.w.assign(w, w + n);
+    W.cbx.assign(n, 0.0); W.cby.assign(n, 0.0);
+    if (cbx) W.cbx.assign(cbx, cbx + n);
+    if (cby) W.cby.assign(cby, cby + n);
+    W.hist.assign(n, {});
+    W.fx.assign(n, 0.0); W.fy.assign(n, 0.0); W.tq.assign(n, 0.0); W.ncont.assign(n, 0);
+    // updateMaximumParticleSize, src/BaseWorld.cpp:201-219
+    if (!(W.cell > 0)) {
+        double mx = 0.0;
+        for (int i = 0; i < n; ++i) if (W.r[i] > mx) mx = W.r[i];
+        W.cell = mx * 2.01;
+    }
+    return XDEM_OK;
+}
+
+int oracle_set_contacts(void* h, int nc, const int* ci, const int* cj,
+                        const double* xix, const double* xiy, const double* nx,
+                        const double* ny, const double* damp) {
+    World& W = *(World*)h;
+    for (int k = 0; k < nc; ++k) {
+        if (ci[k] < 0 || cj[k] <= ci[k] || cj[k] >= W.n) return XDEM_E_INVALID;
+        Contact c;
+        c.xix = xix[k]; c.xiy = xiy[k]; c.nx = nx[k]; c.ny = ny[k]; c.c = damp[k];
+        W.hist[ci[k]][cj[k]] = c;
+    }
+    return XDEM_OK;
+}
+
+int oracle_step(void* h, int nsteps) {
+    World& W = *(World*)h;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = step_once(W);
+        if (rc) return rc;
+    }
+    return XDEM_OK;
+}
+
+int oracle_get_particles(void* h, double* x, double* y, double* th, double* vx,
+                         double* vy, double* w) {
+    World& W = *(World*)h;
+    for (int i = 0; i < W.n; ++i) {
+        x[i] = W.x[i]; y[i] = W.y[i]; th[i] = W.th[i];
+        vx[i] = W.vx[i]; vy[i] = W.vy[i]; w[i] = W.w[i];
+    }
+    return XDEM_OK;
+}
+
+int oracle_get_forces(void* h, double* fx, double* fy, double* tq, int* nct) {
+    World& W = *(World*)h;
+    for (int i = 0; i < W.n; ++i) {
+        fx[i] = W.fx[i]; fy[i] = W.fy[i]; tq[i] = W.tq[i]; nct[i] = W.ncont[i];
+    }
+    return XDEM_OK;
+}
+
+int oracle_num_contacts(void* h) {
+    World& W = *(World*)h;
+    long s = 0;
+    for (auto& m : W.hist) s += (long)m.size();
+    return (int)s;
+}
+
+// live contacts ordered by (i, j)
+int oracle_get_contacts(void* h, int cap, int* nc, int* ci, int* cj,
+                        double* fnx, double* fny, double* fsx, double* fsy,
+                        double* px, double* py, double* xix, double* xiy,
+                        double* nx, double* ny, double* damp, double* t1,
+                        double* t2) {
+    World& W = *(World*)h;
+    int k = 0;
+    for (int i = 0; i < W.n; ++i) {
+        for (auto& kv : W.hist[i]) {
+            if (k < cap) {
+                const Contact& c = kv.second;
+                if (ci) ci[k] = i;
+                if (cj) cj[k] = kv.first;
+                if (fnx) fnx[k] = c.fnx;
+                if (fny) fny[k] = c.fny;
+                if (fsx) fsx[k] = c.fsx;
+                if (fsy) fsy[k] = c.fsy;
+                if (px) px[k] = c.px;
+                if (py) py[k] = c.py;
+                if (xix) xix[k] = c.xix;
+                if (xiy) xiy[k] = c.xiy;
+                if (nx) nx[k] = c.nx;
+                if (ny) ny[k] = c.ny;
+                if (damp) damp[k] = c.c;
+                if (t1) t1[k] = c.t1;
+                if (t2) t2[k] = c.t2;
+            }
+            ++k;
+        }
+    }
+    *nc = k;
+    return XDEM_OK;
+}
+
+// BaseWorld::updateTotalStress, src/BaseWorld.cpp:158-176, with
+// BaseParticle::getStressFromContactList (src/BaseParticle.cpp:143-151) and
+// GluedParticle::getStressFromContactList (src/GluedParticle.cpp:127-139).
+int oracle_get_stress(void* h, double* out) {
+    World& W = *(World*)h;
+    double sxx = 0, sxy = 0, syx = 0, syy = 0;
+    for (int p = 0; p < W.n; ++p) {
+        if (W.flags[p] & SKIP_MACRO_SUM) continue;
+        double txx = 0, txy = 0, tyx = 0, tyy = 0;
+        if (W.kind[p] == KIND_CLUMP) {
+            for (int s = p + 1; s < W.n && W.up[s] == p; ++s) {
+                for (auto& kv : W.hist[s]) {
+                    const Contact& c = kv.second;
+                    double bx = c.bx + W.cbx[s], by = c.by + W.cby[s];
+                    txx += c.fx * bx; txy += c.fx * by; tyx += c.fy * bx; tyy += c.fy * by;
+                }
+            }
+        } else {
+            for (auto& kv : W.hist[p]) {
+                const Contact& c = kv.second;
+                txx += c.fx * c.bx; txy += c.fx * c.by; tyx += c.fy * c.bx; tyy += c.fy * c.by;
+            }
+        }
+        sxx += txx; sxy += txy; syx += tyx; syy += tyy;
+    }
+    double area = W.xlen * W.ylen;
+    out[0] = sxx / area; out[1] = sxy / area; out[2] = syx / area; out[3] = syy / area;
+    return XDEM_OK;
+}
+
+int oracle_get_boundary(void* h, double* out) {
+    World& W = *(World*)h;
+    out[0] = W.shift; out[1] = W.shift_vel; out[2] = W.strain;
+    return XDEM_OK;
+}
+
+int oracle_get_counters(void* h, long long* newton, long long* rescues,
+                        long long* cand, long long* coarse, long long* steps) {
+    World& W = *(World*)h;
+    *newton = W.newton_iters; *rescues = W.rescues; *cand = W.candidates;
+    *coarse = W.coarse; *steps = W.steps;
+    return XDEM_OK;
+}
+
+// Single-pair ZPM/circle geometry (no history), for per-function checks.
+// out: [contact, overlap, px, py, nx, ny, b1x, b1y, b2x, b2y, newton_iters]
+int oracle_pair_geometry(int kind, double r1, double ar1, double x1, double y1,
+                         double th1, double r2, double ar2, double x2, double y2,
+                         double th2, double* out) {
+    World W;
+    W.n = 2;
+    W.kind = {kind, kind}; W.up = {0, 1}; W.flags = {0, 0};
+    W.r = {r1, r2}; W.ar = {ar1, ar2}; W.m = {1, 1}; W.im = {1, 1};
+    W.in = {1, 1}; W.iin = {1, 1};
+    W.x = {x1, x2}; W.y = {y1, y2}; W.th = {th1, th2};
+    W.vx = {0, 0}; W.vy = {0, 0}; W.w = {0, 0};
+    double vxb[2] = {0, 0};
+    Geometry g{};
+    Counters K;
+    int hit;
+    if (kind == KIND_CIRCLE) hit = contact_circles(W, 0, 1, 0, 0, 0, 0, vxb, g) ? 1 : 0;
+    else hit = contact_ellipses(W, 0, 1, 0, 0, 0, 0, vxb, g, K);
+    if (hit < 0) return hit;
+    out[0] = hit; out[1] = g.overlap; out[2] = g.px; out[3] = g.py;
+    out[4] = g.nx; out[5] = g.ny; out[6] = g.b1x; out[7] = g.b1y;
+    out[8] = g.b2x; out[9] = g.b2y; out[10] = (double)K.newton;
+    return XDEM_OK;
+}
+
+int oracle_newton(double A, double B, double C, double D, double E, double init,
+                  int cap, double* q, long* iters) {
+    *q = newton_quartic(A, B, C, D, E, init, cap, iters);
+    return *iters < 0 ? XDEM_E_NEWTON_CAP : XDEM_OK;
+}
+
+int oracle_bisect(double A, double B, double C, double D, double E, double s,
+                  double* q) {
+    return quartic_bisect(A, B, C, D, E, s, q) ? XDEM_OK : XDEM_E_NEWTON_CAP;
+}
+
+}  // extern "C"

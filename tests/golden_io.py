@@ -1,0 +1,31 @@
+"""Loading of the golden fixtures written by tests/golden/make_golden.py."""
+import os
+
+import numpy as np
+
+from xdem_amd.state import State, WorldConfig, _CFG_KEYS
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+WORLD_CASES = ("le_poly", "pb_ellipse", "pb_circle", "pb_ar1", "pb_clumps", "le_poly_hist")
+DUMP_STEPS = (1, 2, 4, 8)
+
+
+def load_world(name):
+    z = np.load(os.path.join(GOLDEN, f"world_{name}.npz"))
+    cfgv = z["in_cfg"]
+    cfg = WorldConfig(world_kind=int(cfgv[0]), **{k: float(v) for k, v in zip(_CFG_KEYS, cfgv[1:])})
+    f = lambda k: z["in_" + k].copy()
+    s = State(cfg, *(f(k) for k in ("kind", "up_level", "flags", "r", "aspect", "mass",
+                                     "inertia", "x", "y", "theta", "vx", "vy", "omega",
+                                     "clump_bx", "clump_by", "ci", "cj", "xi_x", "xi_y",
+                                     "n_x", "n_y", "damping")))
+    steps = {}
+    for st in DUMP_STEPS:
+        pre = f"s{st}_"
+        steps[st] = {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
+    return s, steps
+
+
+def load_pairs():
+    z = np.load(os.path.join(GOLDEN, "pairs.npz"))
+    return z["poses"], z["geometry"]

@@ -1,0 +1,66 @@
+"""The CPU oracle against golden vectors produced by the real reference.
+
+tests/golden/ holds outputs of the reference compiled from its own sources
+(tests/golden/make_golden.py drives oracle/_ref/ref_harness).  Same libm,
+same operation order, so:
+  * single-pair geometry (calculateContactGeometry) is bit-identical;
+  * after step 1 every per-contact quantity is bit-identical; per-particle
+    sums differ only by summation order (unordered_map iteration in the
+    reference), <= 1e-15 of the force scale;
+  * through step 8, contact sets stay identical and every quantity within
+    1e-12 of its scale.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.golden_io import DUMP_STEPS, WORLD_CASES, load_pairs, load_world
+
+FIELDS = ("fn_x", "fn_y", "fs_x", "fs_y", "xi_x", "xi_y", "n_x", "n_y", "point_x",
+          "point_y", "torque_i", "torque_j", "damping")
+
+
+def test_pair_geometry_bitwise():
+    poses, geo = load_pairs()
+    assert poses.shape[0] == 3000
+    for r, g in zip(poses, geo):
+        o = po.pair_geometry(int(r[0]), *r[1:11])
+        assert o[0] == g[0]
+        if g[0]:
+            assert np.array_equal(o[1:10], g[1:10])
+
+
+@pytest.mark.parametrize("name", WORLD_CASES)
+def test_world_against_reference(name):
+    s, steps = load_world(name)
+    w = po.OracleWorld(s)
+    done = 0
+    for st in DUMP_STEPS:
+        w.step(st - done)
+        done = st
+        g = steps[st]
+        c, f, p = w.contacts(), w.forces(), w.particles()
+        assert np.array_equal(c["i"], g["i"]) and np.array_equal(c["j"], g["j"]), \
+            f"{name}: contact set differs at step {st}"
+        fscale = float(np.max(np.abs(g["fn_x"])))
+        tol = 1e-15 if st == 1 else 1e-12
+        if st == 1:
+            for k in FIELDS:
+                assert np.array_equal(c[k], g[k]), f"{name}: {k} not bit-identical at step 1"
+        else:
+            for k in FIELDS:
+                scale = max(float(np.max(np.abs(g[k]))), 1e-300)
+                assert float(np.max(np.abs(c[k] - g[k]))) <= 1e-12 * scale, f"{name}: {k} step {st}"
+        assert np.array_equal(f["num_contacts"], g["num_contacts"])
+        for k in ("fx", "fy"):
+            assert float(np.max(np.abs(f[k] - g[k]))) <= tol * fscale, f"{name}: {k} step {st}"
+        tscale = max(float(np.max(np.abs(g["torque_i"]))), 1e-300)
+        assert float(np.max(np.abs(f["torque"] - g["torque"]))) <= tol * tscale
+        for k in ("x", "y", "theta"):
+            assert float(np.max(np.abs(p[k] - g[k]))) <= 1e-13, f"{name}: {k} step {st}"
+        for k in ("vx", "vy", "omega"):
+            scale = max(float(np.max(np.abs(g[k]))), 1e-300)
+            assert float(np.max(np.abs(p[k] - g[k]))) <= 1e-12 * scale, f"{name}: {k} step {st}"
+        sg = g["stress"]
+        assert float(np.max(np.abs(w.stress() - sg))) <= 1e-12 * float(np.max(np.abs(sg)))
+        assert np.array_equal(w.boundary(), g["boundary"]), f"{name}: LE scalars step {st}"

@@ -1,0 +1,305 @@
+"""Checks of the CPU oracle against what the mathematics fixes, independent of
+the reference's code: closed-form contact distances, the Perram-Wertheim
+contact function, polynomial roots, brute-force candidate sets, a binary
+collision's restitution and duration, free flight and the LE shift.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import helpers
+from xdem_amd.state import (KIND_CIRCLE, KIND_ELLIPSE, WORLD_LEES_EDWARDS, WORLD_PERIODIC,
+                            State, WorldConfig, particle_properties)
+
+
+# ---------------------------------------------------------------------------
+# Perram & Wertheim (J. Comput. Phys. 58, 409 (1985)): for ellipses with
+# quadratic forms A, B and centre offset d, F(l) = l(1-l) d^T [(1-l)A^-1 +
+# l B^-1]^-1 d; the ellipses overlap iff max_l F < 1, and scaling d by s
+# scales F by s^2, so the distance of closest approach along d is
+# |d| / sqrt(max F).
+def _form(a, b, th):
+    c, s = math.cos(th), math.sin(th)
+    R = np.array([[c, -s], [s, c]])
+    return R @ np.diag([1.0 / a**2, 1.0 / b**2]) @ R.T
+
+
+def pw_distance(a1, ar1, th1, a2, ar2, th2, d):
+    Ai = np.linalg.inv(_form(a1, a1 / ar1, th1))
+    Bi = np.linalg.inv(_form(a2, a2 / ar2, th2))
+    F = lambda l: l * (1 - l) * d @ np.linalg.solve((1 - l) * Ai + l * Bi, d)
+    lo, hi = 0.0, 1.0
+    g = (math.sqrt(5) - 1) / 2
+    x1, x2 = hi - g * (hi - lo), lo + g * (hi - lo)
+    f1, f2 = F(x1), F(x2)
+    for _ in range(200):
+        if f1 < f2:
+            lo, x1, f1 = x1, x2, f2
+            x2 = lo + g * (hi - lo)
+            f2 = F(x2)
+        else:
+            hi, x2, f2 = x2, x1, f1
+            x1 = hi - g * (hi - lo)
+            f1 = F(x1)
+    fmax = max(f1, f2)
+    return float(np.linalg.norm(d)) / math.sqrt(fmax)
+
+
+def oracle_distance(a1, ar1, th1, a2, ar2, th2, d):
+    """dist = |d| + overlap0, recovered from normalOverlap = |d_hat . n| overlap0."""
+    o = po.pair_geometry(KIND_ELLIPSE, a1, ar1, 0.0, 0.0, th1, a2, ar2, d[0], d[1], th2)
+    if o[0] == 0:
+        return None, o
+    dl = float(np.linalg.norm(d))
+    proj = abs((d[0] * o[4] + d[1] * o[5]) / dl)
+    return dl + o[1] / proj, o
+
+
+@pytest.mark.parametrize("rot", [0.3, 1.1, 2.7])
+def test_closed_form_distances(rot):
+    """Side by side (minor axes) b1+b2, end to end a1+a2, T-junction a1+b2,
+    for a generic global rotation (exactly axis-aligned poses divide by zero
+    in the reference's eigenvectors, src/MathMatrix2d.h:130-133)."""
+    a1, ar1, a2, ar2 = 0.5, 1.5, 0.45, 2.0
+    b1, b2 = a1 / ar1, a2 / ar2
+    ux, uy = math.cos(rot), math.sin(rot)
+    cases = [
+        (rot, rot, (-uy, ux), b1 + b2),            # parallel, along minor axes
+        (rot, rot, (ux, uy), a1 + a2),             # parallel, along major axes
+        (rot, rot + math.pi / 2, (ux, uy), a1 + b2),   # T-junction
+    ]
+    for th1, th2, e, expect in cases:
+        for gap in (-1e-3, -1e-6, 1e-6):
+            D = expect + gap
+            d = np.array([e[0] * D, e[1] * D])
+            dist, o = oracle_distance(a1, ar1, th1, a2, ar2, th2, d)
+            if gap > 0:
+                assert o[0] == 0, f"separated pose reported in contact ({expect}, {gap})"
+            else:
+                assert o[0] == 1
+                assert abs(dist - expect) <= 1e-9, (expect, dist)
+
+
+def test_circle_limit():
+    """Aspect ratio 1 takes the delta == 0 branch: overlap = r1 + r2 - |d|."""
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        r1, r2 = rng.uniform(0.3, 0.7, 2)
+        ang, th1, th2 = rng.uniform(0, 2 * math.pi, 3)
+        D = rng.uniform(0.5, 1.0) * (r1 + r2)
+        o = po.pair_geometry(KIND_ELLIPSE, r1, 1.0, 0.0, 0.0, th1, r2, 1.0,
+                             D * math.cos(ang), D * math.sin(ang), th2)
+        assert o[0] == 1
+        assert abs(o[1] - (r1 + r2 - D)) <= 1e-12
+        # normal along the centre line
+        assert abs(o[4] - math.cos(ang)) <= 1e-12 and abs(o[5] - math.sin(ang)) <= 1e-12
+
+
+def transformed_eigenvalues(a1, ar1, th1, a2, ar2, th2):
+    """Eigenvalues of ellipse 2's quadratic form after the map that turns
+    ellipse 1 into the unit circle (lengths in units of b1), by numpy."""
+    b1 = a1 / ar1
+    c, s = math.cos(th1), math.sin(th1)
+    R = np.array([[c, -s], [s, c]])
+    Tinv = R @ np.diag([a1 / b1, 1.0]) @ R.T          # circle -> ellipse 1, in b1 units
+    B = _form(a2, a2 / ar2, th2) * b1 * b1
+    return np.linalg.eigvalsh(Tinv.T @ B @ Tinv)
+
+
+def test_distance_against_perram_wertheim():
+    """Random poses: contact decision and distance of closest approach.
+
+    The reference orders the two eigenvalues of the transformed ellipse by
+    comparing them truncated to int (abs() in src/MathMatrix2d.h:122 is C's
+    int abs).  When both truncate to the same integer the transformed semi-axes
+    are swapped: the distances it still reports are right, but the
+    transformed-frame culls (src/EllipticParticle.cpp:77-84) then reject some
+    real overlaps.  Correctly ordered poses must agree with Perram-Wertheim
+    exactly; swapped poses may only miss contacts, never invent them."""
+    rng = np.random.default_rng(2)
+    checked = swapped = missed = 0
+    for _ in range(1500):
+        a1, a2 = rng.uniform(0.4, 0.6, 2)
+        ar1, ar2 = rng.uniform(1.05, 2.5, 2)
+        th1, th2, ang = rng.uniform(0, 2 * math.pi, 3)
+        D = rng.uniform(0.3, 1.0) * (a1 + a2)
+        d = np.array([D * math.cos(ang), D * math.sin(ang)])
+        ev = transformed_eigenvalues(a1, ar1, th1, a2, ar2, th2)
+        if np.min(np.abs(ev - np.round(ev))) < 1e-9:
+            continue                      # truncation undecidable at this precision
+        ref = pw_distance(a1, ar1, th1, a2, ar2, th2, d)
+        if abs(ref - D) < 1e-6 * D:
+            continue                      # on the contact boundary
+        dist, o = oracle_distance(a1, ar1, th1, a2, ar2, th2, d)
+        is_swapped = int(ev[0]) == int(ev[1])
+        swapped += is_swapped
+        if o[0] == 1:
+            assert ref > D, ("false contact", a1, ar1, th1, a2, ar2, th2, d)
+            assert abs(dist - ref) <= 1e-7 * ref
+            checked += 1
+        elif ref > D:
+            assert is_swapped, ("missed contact in a correctly ordered pose",
+                                a1, ar1, th1, a2, ar2, th2, d)
+            missed += 1
+    assert checked > 400 and swapped > 50
+    assert 0 < missed < swapped
+
+
+def test_newton_cycle_rescue():
+    """A pose (from a Lees-Edwards packing) where the eigenvalue order of
+    eigen() swaps the transformed axes and the reference's Newton loop cycles
+    forever: the oracle stops at 200 updates and bisects."""
+    pose = [0.4321304017550254, 1.7999892628071283, -1.7702608555070118, 6.258542045844655,
+            0.18454795164527532, 0.5624894201105685, 1.2857174827581246,
+            -2.1527440943683214, 6.634985207575501, 2.5352734119304268]
+    o = po.pair_geometry(KIND_ELLIPSE, *pose)
+    assert o[0] == 1 and o[10] == 200
+    d = np.array([pose[7] - pose[2], pose[8] - pose[3]])
+    dist, _ = oracle_distance(pose[0], pose[1], pose[4], pose[5], pose[6], pose[9], d)
+    assert abs(dist - pw_distance(pose[0], pose[1], pose[4], pose[5], pose[6], pose[9], d)) <= 1e-9
+
+
+def test_newton_and_bisection_roots():
+    """Newton from the given start and the bracket bisection land on a true
+    root of the quartic (companion-matrix eigenvalues)."""
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        roots = np.sort(rng.uniform(-2, 3, 4))
+        A = -rng.uniform(0.2, 2.0)
+        c = np.poly(roots) * A
+        init = roots[2] + 0.05 * rng.uniform(-1, 1)
+        q, it, rc = po.newton(*c, init)
+        assert rc == 0 and 0 < it < 200
+        assert np.min(np.abs(roots - q)) <= 1e-9
+    # bisection: root r in [lo, hi] = [min(1,s), max(1,s)]
+    for _ in range(100):
+        s = rng.uniform(0.3, 2.0)
+        lo, hi = min(1.0, s), max(1.0, s)
+        r = rng.uniform(lo, hi)
+        others = rng.uniform(-3, -0.5, 3)
+        c = np.poly(np.concatenate([[r], others])) * -1.0
+        q, rc = po.bisect(*c, s)
+        assert rc == 0 and abs(q - r) <= 1e-12
+
+
+def _two_circles(v0, e, kn=1e4, r=0.5):
+    m, inertia, _ = particle_properties([KIND_CIRCLE] * 2, [r, r], [1.0, 1.0])
+    L = 20.0
+    cfg = WorldConfig(world_kind=WORLD_PERIODIC, xmin=-L / 2, xmax=L / 2, ymin=-L / 2,
+                      ymax=L / 2, dt=0.01 * math.sqrt(m[0] / kn), kn=kn, kt=0.5 * kn,
+                      restitution=e, mu=0.5)
+    z = np.zeros(2)
+    return State(cfg, np.zeros(2, np.int32), np.arange(2, dtype=np.int32),
+                 np.zeros(2, np.int32), np.array([r, r]), np.ones(2), m, inertia,
+                 np.array([-r - 1e-3, r + 1e-3]), z.copy(), z.copy(),
+                 np.array([v0, -v0]), z.copy(), z.copy(), z.copy(), z.copy())
+
+
+@pytest.mark.parametrize("e", [0.5, 0.9])
+def test_binary_collision_restitution(e):
+    """Head-on collision of equal disks: outgoing/incoming speed = e and the
+    contact lasts pi / sqrt(kn/m0 - (c/2m0)^2)
+    (src/ContactModel_LinearDissipativeFrictional.cpp:48-54)."""
+    v0 = 0.5
+    s = _two_circles(v0, e)
+    w = po.OracleWorld(s)
+    steps_in_contact = 0
+    for _ in range(20000):
+        w.step(1)
+        if w.contacts()["i"].size:
+            steps_in_contact += 1
+        elif steps_in_contact:
+            break
+    v = w.particles()["vx"]
+    assert abs((v[1] - v[0]) / (2 * v0) - e) <= 0.01
+    m0 = s.mass[0] / 2
+    le = math.log(e)
+    c = math.sqrt(m0) * 2 * math.sqrt(s.cfg.kn) * abs(le) / math.sqrt(math.pi**2 + le * le)
+    tc = math.pi / math.sqrt(s.cfg.kn / m0 - (c / (2 * m0)) ** 2)
+    assert abs(steps_in_contact * s.cfg.dt - tc) <= 3 * s.cfg.dt
+
+
+def test_coulomb_cap():
+    """A sliding contact carries |Fs| = mu |Fn|."""
+    s = _two_circles(0.0, 0.5)
+    s.x[:] = [-0.49, 0.49]
+    s.vy[:] = [5.0, -5.0]
+    w = po.OracleWorld(s)
+    w.step(3)
+    c = w.contacts()
+    fn = math.hypot(c["fn_x"][0], c["fn_y"][0])
+    fs = math.hypot(c["fs_x"][0], c["fs_y"][0])
+    assert abs(fs - s.cfg.mu * fn) <= 1e-12 * fn
+
+
+def _brute_force_pairs(s, le):
+    L, H = s.cfg.xmax - s.cfg.xmin, s.cfg.ymax - s.cfg.ymin
+    S = s.cfg.total_shift
+    pairs = set()
+    for i in range(s.n):
+        for j in range(i + 1, s.n):
+            for m in (-1, 0, 1):
+                for k in (-1, 0, 1):
+                    ox = k * L + (m * S if le else 0.0)
+                    oy = m * H
+                    dx = s.x[j] + ox - s.x[i]
+                    dy = s.y[j] + oy - s.y[i]
+                    if math.hypot(dx, dy) >= s.r[i] + s.r[j]:
+                        continue
+                    o = po.pair_geometry(int(s.kind[i]), s.r[i], s.aspect[i], s.x[i], s.y[i],
+                                         s.theta[i], s.r[j], s.aspect[j], s.x[j] + ox,
+                                         s.y[j] + oy, s.theta[j])
+                    if o[0]:
+                        pairs.add((i, j))
+    return pairs
+
+
+@pytest.mark.parametrize("name", ["le_poly", "pb_ellipse", "pb_circle"])
+def test_contact_set_against_brute_force(name):
+    """Cell binning + 3x3 stencil + image offsets find exactly the contacts of
+    an O(N^2) search over all periodic / Lees-Edwards images."""
+    s = helpers.state(name)
+    s.vx[:] = 0
+    s.vy[:] = 0
+    s.omega[:] = 0
+    le = s.cfg.world_kind == WORLD_LEES_EDWARDS
+    # positions inside the box so the step's wrap is the identity
+    w = po.OracleWorld(s)
+    w.step(1)
+    got = set(zip(w.contacts()["i"].tolist(), w.contacts()["j"].tolist()))
+    assert got == _brute_force_pairs(s, le)
+
+
+def test_free_flight_and_damping():
+    """No contacts: x += v dt exactly; global damping scales v by
+    (1 - g dt)/(1 + g dt) per step (src/BaseParticle.cpp:112-117)."""
+    s = _two_circles(0.3, 0.5)
+    s.x[:] = [-5.0, 5.0]
+    s.vy[:] = [0.1, -0.2]
+    w = po.OracleWorld(s)
+    w.step(1)
+    p = w.particles()
+    assert p["x"][0] == -5.0 + 0.3 * s.cfg.dt and p["y"][1] == 0.0 + (-0.2) * s.cfg.dt
+    s.cfg.global_damping = 3.0
+    w = po.OracleWorld(s)
+    w.step(1)
+    g, dt = 3.0, s.cfg.dt
+    assert abs(w.particles()["vx"][0] - 0.3 * (1 - g * dt) / (1 + g * dt)) <= 1e-16
+
+
+def test_lees_edwards_shift():
+    """S advances by rate * Ly * dt per step and wraps into [xmin, xmax]
+    (src/LeesEdwardsBoundaryWorld.cpp:285-296); strain by rate * dt."""
+    s = helpers.state("le_poly")
+    s.cfg.shear_rate = 50.0
+    s.cfg.total_shift = 0.0
+    w = po.OracleWorld(s)
+    nsteps = 40
+    w.step(nsteps)
+    H = s.cfg.ymax - s.cfg.ymin
+    L = s.cfg.xmax - s.cfg.xmin
+    expect = (50.0 * H * s.cfg.dt * nsteps + s.cfg.xmax) % L - s.cfg.xmax
+    b = w.boundary()
+    assert abs(b[0] - expect) <= 1e-9 and abs(b[2] - 50.0 * s.cfg.dt * nsteps) <= 1e-12

@@ -1,0 +1,266 @@
+// dem_math.h -- device-side contact mathematics of the DEM step engine.
+//
+// Every floating-point expression keeps the evaluation order of the
+// reference (Xiaoyu-JIANG/xDEM, src/), and the library is compiled with
+// -ffp-contract=off, so the FP64 results agree with the reference's
+// -O3/no-FMA build wherever the math library calls (sin, cos, atan2) agree.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xdem {
+
+enum : int { KIND_CIRCLE = 0, KIND_ELLIPSE = 1, KIND_CLUMP = 2 };
+enum : int { SKIP_TIME_INTEGRAL = 1, SKIP_CONTACT_DETECTION = 2, SKIP_MACRO_SUM = 4 };
+enum : int { WORLD_PERIODIC = 0, WORLD_LE = 1 };
+
+// error bits raised by kernels (mapped to XDEM_E_* on the host)
+enum : uint32_t {
+    ERR_CAND_OVERFLOW = 1u << 0,   // >= 200 candidates (reference exit(-2))
+    ERR_UNSUPPORTED = 1u << 1,     // unlike shapes (reference exit(-1))
+    ERR_ZERO_NORMAL = 1u << 2,     // normalize() of zero vector
+    ERR_NEWTON_CAP = 1u << 3,      // Newton cap, no rescue possible or strict
+    ERR_BIN_OVERFLOW = 1u << 4,    // engine bin capacity
+    ERR_PAIR_OVERFLOW = 1u << 5,   // engine pair/contact table capacity
+    ERR_INCOMING_OVERFLOW = 1u << 6,
+    ERR_ROW_OVERFLOW = 1u << 7,    // engine per-particle pair capacity
+};
+
+struct Geom {                       // src/ContactGeometry.h:5-16
+    double px, py, nx, ny, b1x, b1y, b2x, b2y, overlap;
+};
+
+// abs((int)v) as the reference's x86-64 build evaluates it inside
+// matrix2d::eigen (src/MathMatrix2d.h:122): that header is parsed before any
+// declaration of abs(double), so abs is C's int abs(int).  The conversion is
+// truncation toward zero, and NaN / out-of-range give INT_MIN (cvttsd2si),
+// whose abs stays INT_MIN.
+__device__ __forceinline__ long long int_abs_of(double v) {
+    const double lim = 2147483648.0;
+    if (!(v > -lim - 1.0 && v < lim)) return -2147483648LL;
+    long long k = (long long)v;
+    return k < 0 ? -k : k;
+}
+
+// Newton iteration of EllipticParticle::NewtonRaphsonForQuartic
+// (src/EllipticParticle.cpp:613-626), capped at `cap` updates (the reference
+// is uncapped).  Returns the update count or -1 when the cap was reached.
+__device__ __forceinline__ int newton_quartic(double A, double B, double C,
+                                              double D, double E, double init,
+                                              int cap, double* root) {
+    const double A0 = 4.0 * A;
+    const double B0 = 3.0 * B;
+    const double C0 = 2.0 * C;
+    double r0 = init;
+    double r = 0.0;
+    int it = 0;
+    while (fabs(r0 - r) > 1e-10) {
+        if (it >= cap) { *root = r0; return -1; }
+        r = r0;
+        const double p2 = r0 * r0;
+        const double p3 = r0 * r0 * r0;
+        const double p4 = r0 * r0 * r0 * r0;
+        const double num = A * p4 + B * p3 + C * p2 + D * r0 + E;
+        const double den = A0 * p3 + B0 * p2 + C0 * r0 + D;
+        r0 -= num / den;
+        ++it;
+    }
+    *root = r0;
+    return it;
+}
+
+// Rescue for a Newton run that reached the cap: 64 bisections of
+// [min(1,s), max(1,s)], s = sqrt(delta+1); the root q satisfies
+// sin^2(psi) = (q^2-1)/delta in [0,1] (src/EllipticParticle.cpp:167,204).
+__device__ __forceinline__ bool quartic_bisect(double A, double B, double C,
+                                               double D, double E, double s,
+                                               double* root) {
+    double lo = s < 1.0 ? s : 1.0, hi = s < 1.0 ? 1.0 : s;
+    double flo = (((A * lo + B) * lo + C) * lo + D) * lo + E;
+    const double fhi = (((A * hi + B) * hi + C) * hi + D) * hi + E;
+    if (flo == 0.0) { *root = lo; return true; }
+    if (fhi == 0.0) { *root = hi; return true; }
+    if ((flo > 0.0) == (fhi > 0.0) || !(flo == flo) || !(fhi == fhi)) return false;
+    for (int k = 0; k < 64; ++k) {
+        const double m = 0.5 * (lo + hi);
+        const double fm = (((A * m + B) * m + C) * m + D) * m + E;
+        if ((fm > 0.0) == (flo > 0.0)) { lo = m; flo = fm; }
+        else hi = m;
+    }
+    *root = 0.5 * (lo + hi);
+    return true;
+}
+
+// Quantities of one ellipse pair up to the quartic (ZPM, eqs 4-26 of Zheng
+// & Palffy-Muhoray, Phys. Rev. E 75, 061709), src/EllipticParticle.cpp:49-100.
+struct ZpmSetup {
+    double b1, e1s, b1s, dlen, dhx, dhy, k1x, k1y;
+    double Txx, Txy, Tyx, Tyy;
+    double kpx, kpy, kmx, kmy, a2p, b2p, delta;
+    double A, B, C, D, E;
+};
+
+// Returns 0 when one of the transformed-frame culls (src/EllipticParticle.cpp:77-84)
+// rejects the pair, 1 when the quartic is ready.
+__device__ __forceinline__ int zpm_setup(double a1, double ar1, double z1,
+                                         double a2, double ar2, double z2,
+                                         double dx, double dy, ZpmSetup& S) {
+    const double b1 = a1 / ar1, b2 = a2 / ar2;
+    const double dlen = sqrt(dx * dx + dy * dy);
+    const double dhx = dx / dlen, dhy = dy / dlen;
+    double s1z, c1z, s2z, c2z;
+    sincos(z1, &s1z, &c1z);
+    sincos(z2, &s2z, &c2z);
+    const double k1x = c1z, k1y = s1z, k2x = c2z, k2y = s2z;
+    const double kxx = k1x * k1x, kxy = k1x * k1y, kyx = k1y * k1x, kyy = k1y * k1y;
+    const double qxx = k2x * k2x, qxy = k2x * k2y, qyx = k2y * k2x, qyy = k2y * k2y;
+    const double e1 = sqrt(1.0 - (b1 * b1) / (a1 * a1));
+    const double e2 = sqrt(1.0 - (b2 * b2) / (a2 * a2));
+    // T = (I + k1k1*(b1/a1-1)) / b1
+    const double s1 = b1 / a1 - 1.0;
+    const double Txx = (1.0 + kxx * s1) / b1, Txy = (0.0 + kxy * s1) / b1;
+    const double Tyx = (0.0 + kyx * s1) / b1, Tyy = (1.0 + kyy * s1) / b1;
+    const double eta = a1 / b1 - 1.0;
+    const double dphx = Txx * dhx + Txy * dhy, dphy = Tyx * dhx + Tyy * dhy;
+    const double dp0x = Txx * dx + Txy * dy, dp0y = Tyx * dx + Tyy * dy;
+    // A' = (I+k1k1*eta)(I-k2k2*e2^2)(I+k1k1*eta) * b1^2 / b2^2, with the
+    // product of src/MathMatrix2d.h:85-91 (its YY term uses o.yx)
+    const double m1xx = 1.0 + kxx * eta, m1xy = 0.0 + kxy * eta;
+    const double m1yx = 0.0 + kyx * eta, m1yy = 1.0 + kyy * eta;
+    const double e2s = e2 * e2;
+    const double mbxx = 1.0 - qxx * e2s, mbxy = 0.0 - qxy * e2s;
+    const double mbyx = 0.0 - qyx * e2s, mbyy = 1.0 - qyy * e2s;
+    const double pxx = m1xx * mbxx + m1xy * mbyx, pxy = m1xx * mbxy + m1xy * mbyy;
+    const double pyx = m1yx * mbxx + m1yy * mbyx, pyy = m1yx * mbyx + m1yy * mbyy;
+    double axx = pxx * m1xx + pxy * m1yx, axy = pxx * m1xy + pxy * m1yy;
+    double ayx = pyx * m1xx + pyy * m1yx, ayy = pyx * m1yx + pyy * m1yy;
+    const double b1s = b1 * b1, b2s = b2 * b2;
+    axx = axx * b1s; axy = axy * b1s; ayx = ayx * b1s; ayy = ayy * b1s;
+    axx = axx / b2s; axy = axy / b2s; ayx = ayx / b2s; ayy = ayy / b2s;
+    // matrix2d::eigen (src/MathMatrix2d.h:112-135)
+    const double tr = axx + ayy;
+    const double det = axx * ayy - axy * ayx;
+    double disc = tr * tr - 4.0 * det;
+    if (disc < 0) disc = 0.0;
+    disc = sqrt(disc);
+    const double ev1 = (tr + disc) / 2.0, ev2 = (tr - disc) / 2.0;
+    double lx, ly;
+    if (int_abs_of(ev1) > int_abs_of(ev2)) { lx = ev1; ly = ev2; }
+    else { lx = ev2; ly = ev1; }
+    double kpx = 1.0, kpy = (lx - axx) / axy;
+    {
+        const double ls = kpx * kpx + kpy * kpy;
+        const double s = sqrt(ls);     // ls >= 1 or NaN: never zero
+        kpx /= s; kpy /= s;
+    }
+    double kmx = 1.0, kmy = (ly - axx) / axy;
+    {
+        const double ls = kmx * kmx + kmy * kmy;
+        const double s = sqrt(ls);
+        kmx /= s; kmy /= s;
+    }
+    const double a2p = 1.0 / sqrt(ly);
+    const double b2p = 1.0 / sqrt(lx);
+    if (sqrt(dp0x * dp0x + dp0y * dp0y) > 1.0 + a2p) return 0;
+    if (fabs(dp0x * kpx + dp0y * kpy) > 1.0 + b2p) return 0;
+    if (kmx * dphx + kmy * dphy < 0) { kmx = -kmx; kmy = -kmy; }
+    if (kpx * dphx + kpy * dphy < 0) { kpx = -kpx; kpy = -kpy; }
+    const double delta = (a2p * a2p) / (b2p * b2p) - 1;
+    const double tanphi = (kmx * dphx + kmy * dphy) / (kpx * dphx + kpy * dphy);
+    const double t2 = tanphi * tanphi;
+    S.A = -(1.0 + t2) / (b2p * b2p);
+    S.B = -2.0 * (1.0 + t2 + delta) / b2p;
+    S.C = -t2 - (1.0 + delta) * (1.0 + delta) + (1.0 + (1.0 + delta) * t2) / (b2p * b2p);
+    S.D = 2.0 * (1.0 + t2) * (1.0 + delta) / b2p;
+    S.E = (1.0 + t2 + delta) * (1.0 + delta);
+    S.b1 = b1; S.e1s = e1 * e1; S.b1s = b1s; S.dlen = dlen; S.dhx = dhx; S.dhy = dhy;
+    S.k1x = k1x; S.k1y = k1y;
+    S.Txx = Txx; S.Txy = Txy; S.Tyx = Tyx; S.Tyy = Tyy;
+    S.kpx = kpx; S.kpy = kpy; S.kmx = kmx; S.kmy = kmy;
+    S.a2p = a2p; S.b2p = b2p; S.delta = delta;
+    return 1;
+}
+
+// From the quartic root q to the contact geometry, src/EllipticParticle.cpp:166-221.
+// (xi, yi) is P1's centre, (xj2, yj2) the image centre of P2.  Returns 0 no
+// contact, 1 contact, -1 zero normal.
+__device__ __forceinline__ int zpm_finish(const ZpmSetup& S, double q, double xi,
+                                          double yi, double xj2, double yj2,
+                                          double dx, double dy, Geom& g) {
+    const double delta = S.delta;
+    double temp = (q * q - 1.0) / delta;
+    double dprime;
+    if (temp < 0) temp = 0.0;
+    if (delta == 0 || !isfinite(temp)) {
+        dprime = 1.0 + S.a2p;
+    } else {
+        const double u = (1.0 + S.b2p * (1.0 + delta) / q);
+        const double v = (1.0 + S.b2p / q);
+        dprime = sqrt(temp * (u * u) + (1.0 - temp) * (v * v));
+    }
+    const double kd = S.k1x * S.dhx + S.k1y * S.dhy;
+    const double dist = dprime / sqrt(1.0 - (kd * kd) * S.e1s) * S.b1;
+    const double overlap0 = dist - sqrt(dx * dx + dy * dy);
+    if (overlap0 < 0.0) return 0;
+    double wx, wy;
+    if (delta == 0) {
+        wx = S.dhx; wy = S.dhy;
+    } else {
+        double sp2 = (q * q - 1.0) / delta;
+        if (sp2 < 0) sp2 = 0.0;
+        double cp2 = 1.0 - sp2;
+        if (cp2 < 0) cp2 = 0.0;
+        const double sx = sqrt(sp2), sy = sqrt(cp2);
+        wx = S.kmx * sx + S.kpx * sy;
+        wy = S.kmy * sx + S.kpy * sy;
+    }
+    const double tdet = S.Txx * S.Tyy - S.Txy * S.Tyx;
+    const double ixx = S.Tyy / tdet, ixy = -S.Txy / tdet;
+    const double iyx = -S.Tyx / tdet, iyy = S.Txx / tdet;
+    const double tpx = ixx * wx + ixy * wy, tpy = iyx * wx + iyy * wy;
+    const double dl2 = sqrt(dx * dx + dy * dy);
+    const double cpx = tpx * dl2 / dist, cpy = tpy * dl2 / dist;
+    const double e1s = S.e1s, b1s = S.b1s;
+    const double kxx = S.k1x * S.k1x, kxy = S.k1x * S.k1y;
+    const double kyx = S.k1y * S.k1x, kyy = S.k1y * S.k1y;
+    const double Axx = (1.0 - kxx * e1s) / b1s, Axy = (0.0 - kxy * e1s) / b1s;
+    const double Ayx = (0.0 - kyx * e1s) / b1s, Ayy = (1.0 - kyy * e1s) / b1s;
+    double nx = Axx * tpx + Axy * tpy, ny = Ayx * tpx + Ayy * tpy;
+    {
+        const double ls = nx * nx + ny * ny;
+        if (ls != 0.0) { const double s = sqrt(ls); nx /= s; ny /= s; }
+        else return -1;
+    }
+    double ovx = S.dhx * overlap0, ovy = S.dhy * overlap0;
+    const double od = ovx * nx + ovy * ny;
+    ovx = nx * od; ovy = ny * od;
+    g.overlap = sqrt(ovx * ovx + ovy * ovy);
+    g.nx = nx; g.ny = ny;
+    g.px = cpx + xi; g.py = cpy + yi;
+    g.b1x = g.px - xi; g.b1y = g.py - yi;
+    g.b2x = g.px - xj2; g.b2y = g.py - yj2;
+    return 1;
+}
+
+// CircularParticle::contactWithCircularParticle, src/CircularParticle.cpp:67-92.
+__device__ __forceinline__ bool circle_contact(double xi, double yi, double ri,
+                                               double xj2, double yj2, double rj,
+                                               Geom& g) {
+    const double bvx = xj2 - xi, bvy = yj2 - yi;
+    const double distance = sqrt(bvx * bvx + bvy * bvy);
+    const double sum_r = ri + rj;
+    g.overlap = sum_r - distance;
+    if (g.overlap < 0.0) return false;
+    const double ls = bvx * bvx + bvy * bvy;
+    if (ls != 0.0) { const double s = sqrt(ls); g.nx = bvx / s; g.ny = bvy / s; }
+    else { g.nx = 0.0; g.ny = 0.0; }
+    const double p1x = xi + g.nx * ri, p1y = yi + g.ny * ri;
+    const double p2x = xj2 - g.nx * rj, p2y = yj2 - g.ny * rj;
+    g.px = (p1x + p2x) / 2.0;
+    g.py = (p1y + p2y) / 2.0;
+    g.b1x = g.px - xi; g.b1y = g.py - yi;
+    g.b2x = g.px - xj2; g.b2y = g.py - yj2;
+    return true;
+}
+
+}  // namespace xdem

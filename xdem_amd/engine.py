@@ -1,0 +1,218 @@
+"""ctypes binding of libxdem_hip.so (include/xdem.h).
+
+The library is the product path: there is no CPU fallback.  Importing this
+module does not touch the GPU; constructing :class:`Engine` loads the
+library and raises if it is missing or no HIP device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Sequence
+
+import numpy as np
+
+from .state import State, WorldConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxdem_hip.so")
+
+XDEM_OK = 0
+ERRORS = {-1: "XDEM_E_INVALID", -2: "XDEM_E_CAPACITY", -3: "XDEM_E_UNSUPPORTED_PAIR",
+          -4: "XDEM_E_ZERO_NORMAL", -5: "XDEM_E_NONFINITE", -6: "XDEM_E_NEWTON_CAP",
+          -7: "XDEM_E_HIP", -8: "XDEM_E_GRID"}
+
+# exported entry points of include/xdem.h
+SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles",
+           "xdem_set_contacts", "xdem_step", "xdem_sync", "xdem_get_particles",
+           "xdem_get_forces", "xdem_get_contacts", "xdem_get_stress", "xdem_get_boundary",
+           "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
+           "xdem_set_mlp", "xdem_log_roots")
+
+
+class XdemConfig(C.Structure):
+    _fields_ = [("world_kind", C.c_int), ("precision", C.c_int)] + [
+        (k, C.c_double) for k in ("xmin", "xmax", "ymin", "ymax", "dt", "kn", "kt",
+                                  "restitution", "mu", "global_damping", "shear_rate",
+                                  "total_shift", "total_strain", "cell_size")] + [
+        ("newton_cap", C.c_int), ("strict_newton", C.c_int)]
+
+    @classmethod
+    def from_world(cls, c: WorldConfig, precision: int = 0) -> "XdemConfig":
+        return cls(c.world_kind, precision, c.xmin, c.xmax, c.ymin, c.ymax, c.dt, c.kn,
+                   c.kt, c.restitution, c.mu, c.global_damping, c.shear_rate,
+                   c.total_shift, c.total_strain, c.cell_size, c.newton_cap,
+                   c.strict_newton)
+
+
+_D = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_F = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libxdem_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m xdem_amd.build`")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.xdem_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(XdemConfig)]
+    L.xdem_destroy.argtypes = [vp]
+    L.xdem_destroy.restype = None
+    L.xdem_last_error.argtypes = [vp]
+    L.xdem_last_error.restype = C.c_char_p
+    L.xdem_set_particles.argtypes = [vp, C.c_int, vp] + [_I] * 3 + [_D] * 10 + [vp, vp]
+    L.xdem_set_contacts.argtypes = [vp, C.c_int, _I, _I] + [_D] * 5
+    L.xdem_step.argtypes = [vp, C.c_int]
+    L.xdem_sync.argtypes = [vp]
+    L.xdem_get_particles.argtypes = [vp] + [_D] * 6
+    L.xdem_get_forces.argtypes = [vp, _D, _D, _D, _I]
+    L.xdem_get_contacts.argtypes = [vp, C.c_int, C.POINTER(C.c_int)] + [vp] * 15
+    L.xdem_get_stress.argtypes = [vp, _D]
+    L.xdem_get_boundary.argtypes = [vp, _D]
+    L.xdem_get_counters.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 6
+    L.xdem_set_timing.argtypes = [vp, C.c_int]
+    L.xdem_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.xdem_get_traffic.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.xdem_set_mlp.argtypes = [vp] + [_F] * 6
+    L.xdem_log_roots.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _F, _D, _I]
+    _lib = L
+    return L
+
+
+class XdemError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+class Engine:
+    """One libxdem_hip engine holding one or several packings."""
+
+    def __init__(self, states: Sequence[State] | State, precision: int = 0):
+        if isinstance(states, State):
+            states = [states]
+        self.L = load_library()
+        self.states = list(states)
+        k = len(self.states)
+        cfgs = (XdemConfig * k)(*[XdemConfig.from_world(s.cfg, precision) for s in self.states])
+        h = C.c_void_p()
+        rc = self.L.xdem_create(C.byref(h), k, cfgs)
+        if rc != 0:
+            raise XdemError(rc, "xdem_create failed")
+        self.h = h
+        sizes = [s.n for s in self.states]
+        self.starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        self.n = int(self.starts[-1])
+        cat = lambda name: np.concatenate([getattr(s, name) for s in self.states])
+        up = np.concatenate([s.up_level + st for s, st in zip(self.states, self.starts[:-1])])
+        starts = self.starts.copy()
+        self._keep = starts
+        self._check(self.L.xdem_set_particles(
+            self.h, self.n, starts.ctypes.data_as(C.c_void_p), _i32(cat("kind")), _i32(up),
+            _i32(cat("flags")), _f64(cat("r")), _f64(cat("aspect")), _f64(cat("mass")),
+            _f64(cat("inertia")), _f64(cat("x")), _f64(cat("y")), _f64(cat("theta")),
+            _f64(cat("vx")), _f64(cat("vy")), _f64(cat("omega")),
+            _f64(cat("clump_bx")).ctypes.data_as(C.c_void_p),
+            _f64(cat("clump_by")).ctypes.data_as(C.c_void_p)))
+        self._cb = None
+        ci = [s.ci + st for s, st in zip(self.states, self.starts[:-1]) if s.ci.size]
+        if ci:
+            cj = [s.cj + st for s, st in zip(self.states, self.starts[:-1]) if s.ci.size]
+            hist = [np.concatenate([getattr(s, f) for s in self.states if s.ci.size])
+                    for f in ("xi_x", "xi_y", "n_x", "n_y", "damping")]
+            ci, cj = np.concatenate(ci), np.concatenate(cj)
+            self._check(self.L.xdem_set_contacts(self.h, int(ci.size), _i32(ci), _i32(cj),
+                                                 *[_f64(a) for a in hist]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xdem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise XdemError(rc, self.L.xdem_last_error(self.h).decode())
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, nsteps: int = 1, sync: bool = True):
+        self._check(self.L.xdem_step(self.h, int(nsteps)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._check(self.L.xdem_sync(self.h))
+
+    # -- read-back ----------------------------------------------------------
+    def particles(self):
+        out = [np.zeros(self.n) for _ in range(6)]
+        self._check(self.L.xdem_get_particles(self.h, *out))
+        return dict(zip(("x", "y", "theta", "vx", "vy", "omega"), out))
+
+    def forces(self):
+        fx, fy, tq = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        nc = np.zeros(self.n, np.int32)
+        self._check(self.L.xdem_get_forces(self.h, fx, fy, tq, nc))
+        return dict(fx=fx, fy=fy, torque=tq, num_contacts=nc)
+
+    def contacts(self):
+        nc = C.c_int(0)
+        self._check(self.L.xdem_get_contacts(self.h, 0, C.byref(nc), *([None] * 15)))
+        m = nc.value
+        ci, cj = np.zeros(m, np.int32), np.zeros(m, np.int32)
+        names = ("fn_x", "fn_y", "fs_x", "fs_y", "point_x", "point_y", "xi_x", "xi_y",
+                 "n_x", "n_y", "damping", "torque_i", "torque_j")
+        arrs = [np.zeros(m) for _ in names]
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.L.xdem_get_contacts(self.h, m, C.byref(nc), ptr(ci), ptr(cj),
+                                             *[ptr(a) for a in arrs]))
+        d = dict(zip(names, arrs))
+        d["i"], d["j"] = ci, cj
+        return d
+
+    def stress(self):
+        out = np.zeros(4 * len(self.states))
+        self._check(self.L.xdem_get_stress(self.h, out))
+        return out.reshape(-1, 4)
+
+    def boundary(self):
+        out = np.zeros(3 * len(self.states))
+        self._check(self.L.xdem_get_boundary(self.h, out))
+        return out.reshape(-1, 3)
+
+    def counters(self):
+        v = [C.c_longlong(0) for _ in range(6)]
+        self._check(self.L.xdem_get_counters(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("newton_iters", "rescues", "fallbacks", "candidates",
+                         "coarse_pairs", "steps"), [x.value for x in v]))
+
+    def set_timing(self, on: bool):
+        self._check(self.L.xdem_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        ms, n = C.c_double(0), C.c_longlong(0)
+        self._check(self.L.xdem_get_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def traffic(self):
+        b, c = C.c_double(0), C.c_longlong(0)
+        self._check(self.L.xdem_get_traffic(self.h, C.byref(b), C.byref(c)))
+        return b.value, c.value

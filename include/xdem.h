@@ -87,6 +87,8 @@ typedef struct xdem_config {
     int    newton_cap;       /* <= 0: 200 Newton updates per pair */
     int    strict_newton;    /* 1: a pair that reaches the cap fails the step
                                 with XDEM_E_NEWTON_CAP; 0: bisection rescue */
+    double skin;             /* pair-list skin (length); <= 0: 0.2 * max outer
+                                radius.  Results do not depend on it. */
 } xdem_config;
 
 /* Create an engine holding num_packings independent worlds. */
@@ -155,11 +157,12 @@ int xdem_get_boundary(xdem_engine* e, double* out);
 
 /* Counters since creation: Newton iterations, bisection rescues of
  * non-converging Newton runs, fallbacks to the plain root path (NN mode),
- * candidate pairs visited, pairs that passed the bounding-circle test, steps. */
+ * candidate pairs visited at pair-list builds, pairs that passed the
+ * bounding-circle test, steps, pair-list builds.  Any pointer may be NULL. */
 int xdem_get_counters(xdem_engine* e, long long* newton_iters,
                       long long* rescues, long long* fallbacks,
                       long long* candidates, long long* coarse_pairs,
-                      long long* steps);
+                      long long* steps, long long* list_builds);
 
 /* Kernel timing: when enabled, HIP events bracket every launch of the
  * contact kernel on the engine's stream; xdem_get_timing returns the summed

@@ -208,6 +208,9 @@ static inline M2 mul(const M2& a, const M2& o) {
 
 // ZPM ellipse-ellipse contact, src/EllipticParticle.cpp:16-32 and 34-285.
 // Returns 0 no contact, 1 contact, or a negative error code.
+static int zpm_contact(const World& W, int i, int j, double ox, double oy,
+                       Geometry& g, Counters& K, bool fix_order);
+
 static int contact_ellipses(const World& W, int i, int j, double ox, double oy,
                             double ovx, double ovy, const double* vxb,
                             Geometry& g, Counters& K) {
@@ -217,6 +220,25 @@ static int contact_ellipses(const World& W, int i, int j, double ox, double oy,
         if (!((W.r[i] + W.r[j]) > std::sqrt(qx * qx + qy * qy))) return 0;
     }
     K.coarse++;
+    int hit = zpm_contact(W, i, j, ox, oy, g, K, false);
+    // Where the reference produces no valid answer -- a contact whose
+    // geometry is not finite (Newton settled on the root -sqrt(delta+1) of an
+    // axis-swapped pose, so sqrt() of a negative number) -- the pair is
+    // recomputed with the eigenvalues ordered by |lambda| as intended.
+    if (hit == 1 && !(std::isfinite(g.overlap) && std::isfinite(g.nx) && std::isfinite(g.ny) &&
+                      std::isfinite(g.px) && std::isfinite(g.py))) {
+        if (W.strict_newton) return XDEM_E_NEWTON_CAP;
+        K.rescues++;
+        hit = zpm_contact(W, i, j, ox, oy, g, K, true);
+    }
+    if (hit == 1) relative_velocity(W, i, j, ovx, ovy, vxb, g);
+    return hit;
+}
+
+// ZPM distance of closest approach and contact geometry,
+// src/EllipticParticle.cpp:34-285 (without the relative velocity).
+static int zpm_contact(const World& W, int i, int j, double ox, double oy,
+                       Geometry& g, Counters& K, bool fix_order) {
     const double a1 = W.r[i], a2 = W.r[j];
     const double b1 = a1 / W.ar[i], b2 = a2 / W.ar[j];
     const double z1 = W.th[i], z2 = W.th[j];
@@ -258,7 +280,9 @@ static int contact_ellipses(const World& W, int i, int j, double ox, double oy,
         // MathMatrix2d.h is parsed before any header declares abs(double),
         // so its unqualified abs() is C's int abs(int): the eigenvalues are
         // compared after truncation to int (src/MathMatrix2d.h:122).
-        if (int_abs_of(ev1) > int_abs_of(ev2)) { lx = ev1; ly = ev2; }
+        const bool first = fix_order ? std::fabs(ev1) > std::fabs(ev2)
+                                     : int_abs_of(ev1) > int_abs_of(ev2);
+        if (first) { lx = ev1; ly = ev2; }
         else { lx = ev2; ly = ev1; }
         kpx = 1.0; kpy = (lx - Ap.xx) / Ap.xy;
         double ls = kpx * kpx + kpy * kpy;
@@ -348,7 +372,6 @@ static int contact_ellipses(const World& W, int i, int j, double ox, double oy,
     g.px = cpx + W.x[i]; g.py = cpy + W.y[i];
     g.b1x = g.px - W.x[i]; g.b1y = g.py - W.y[i];
     g.b2x = g.px - (W.x[j] + ox); g.b2y = g.py - (W.y[j] + oy);
-    relative_velocity(W, i, j, ovx, ovy, vxb, g);
     return 1;
 }
 

@@ -47,3 +47,13 @@ def test_engine_rejects_bad_config_without_gpu():
     cfg[0].xmin, cfg[0].xmax, cfg[0].ymin, cfg[0].ymax, cfg[0].dt = 0.0, -1.0, 0.0, 1.0, 1e-4
     h = ctypes.c_void_p()
     assert L.xdem_create(ctypes.byref(h), 1, cfg) == -1
+
+
+def test_cpp_adapter_compiles():
+    """The reference-style C++ adapter (xdem_world.hpp) builds and links
+    against the C-ABI library with plain g++ (host code only)."""
+    import subprocess
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "cpp"), "-B"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "world_demo"))

@@ -303,3 +303,36 @@ def test_lees_edwards_shift():
     expect = (50.0 * H * s.cfg.dt * nsteps + s.cfg.xmax) % L - s.cfg.xmax
     b = w.boundary()
     assert abs(b[0] - expect) <= 1e-9 and abs(b[2] - 50.0 * s.cfg.dt * nsteps) <= 1e-12
+
+
+# Overlapping pair from a relaxing 200k-ellipse packing: the axis-swapped
+# Newton run converges to the root -sqrt(delta+1), the reference's distance
+# is sqrt(negative) = NaN and NaN < 0 is false, so it records a contact with
+# NaN force (src/EllipticParticle.cpp:167-186).
+NAN_POSE = [0.42184524440207977, 1.8173639393084935, 48.843723782965725, -45.564079364493296,
+            2.2392434995320016, 0.5940649640548685, 1.4722812591293573, 48.61361788595985,
+            -45.888027843597094, 0.21492088736897055]
+
+
+def test_nonfinite_geometry_is_recomputed():
+    """Where the reference gives no valid answer the oracle recomputes the
+    pair with |lambda|-ordered eigenvalues: the distance then matches
+    Perram-Wertheim and the swapped-role evaluation of the same pair."""
+    o = po.pair_geometry(KIND_ELLIPSE, *NAN_POSE)
+    assert o[0] == 1 and np.all(np.isfinite(o[1:10]))
+    p = NAN_POSE
+    d = np.array([p[7] - p[2], p[8] - p[3]])
+    dist, _ = oracle_distance(p[0], p[1], p[4], p[5], p[6], p[9], d)
+    assert abs(dist - pw_distance(p[0], p[1], p[4], p[5], p[6], p[9], d)) <= 1e-9 * dist
+    swapped = po.pair_geometry(KIND_ELLIPSE, *p[5:10], *p[0:5])
+    assert abs(swapped[1] - o[1]) <= 1e-9
+
+
+def test_reference_returns_nan_for_that_pose(tmp_path):
+    """The real reference (built here from its sources) does return a NaN
+    contact for the pose above."""
+    if not po.have_reference():
+        pytest.skip("reference build not present (oracle/_ref is built where the sources are)")
+    row = np.array([[1.0] + NAN_POSE + [0.0]])
+    g = po.reference_pairs(row, str(tmp_path))[0]
+    assert g[0] == 1.0 and np.isnan(g[1])

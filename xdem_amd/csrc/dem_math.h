@@ -24,6 +24,7 @@ enum : uint32_t {
     ERR_PAIR_OVERFLOW = 1u << 5,   // engine pair/contact table capacity
     ERR_INCOMING_OVERFLOW = 1u << 6,
     ERR_ROW_OVERFLOW = 1u << 7,    // engine per-particle pair capacity
+    ERR_NONFINITE = 1u << 8,       // a particle position became NaN/inf
 };
 
 struct Geom {                       // src/ContactGeometry.h:5-16
@@ -102,9 +103,12 @@ struct ZpmSetup {
 
 // Returns 0 when one of the transformed-frame culls (src/EllipticParticle.cpp:77-84)
 // rejects the pair, 1 when the quartic is ready.
+// fix_order = false orders the eigenvalues as the reference does (through
+// int abs); true orders them by |lambda|, as the algorithm intends.
 __device__ __forceinline__ int zpm_setup(double a1, double ar1, double z1,
                                          double a2, double ar2, double z2,
-                                         double dx, double dy, ZpmSetup& S) {
+                                         double dx, double dy, ZpmSetup& S,
+                                         bool fix_order = false) {
     const double b1 = a1 / ar1, b2 = a2 / ar2;
     const double dlen = sqrt(dx * dx + dy * dy);
     const double dhx = dx / dlen, dhy = dy / dlen;
@@ -145,7 +149,8 @@ __device__ __forceinline__ int zpm_setup(double a1, double ar1, double z1,
     disc = sqrt(disc);
     const double ev1 = (tr + disc) / 2.0, ev2 = (tr - disc) / 2.0;
     double lx, ly;
-    if (int_abs_of(ev1) > int_abs_of(ev2)) { lx = ev1; ly = ev2; }
+    const bool first = fix_order ? fabs(ev1) > fabs(ev2) : int_abs_of(ev1) > int_abs_of(ev2);
+    if (first) { lx = ev1; ly = ev2; }
     else { lx = ev2; ly = ev1; }
     double kpx = 1.0, kpy = (lx - axx) / axy;
     {

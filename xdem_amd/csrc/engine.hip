@@ -2,15 +2,19 @@
 // include/xdem.h.
 //
 // One step of every packing is five launches on the engine's stream:
-//   k_bin         wrap positions (modifyParticlePosition), cell index, fill
-//                 fixed-capacity cell bins and the two ghost rows the
-//                 Lees-Edwards boundary needs (findPossibleContacts binning)
-//   k_candidates  3x3 stencil, candidate filters, image offsets, bounding
-//                 circle test; block-compacted pair list; carries forward
-//                 history entries whose pair is no longer a candidate
-//   k_pairs       one lane per pair: ZPM / circle geometry, history lookup
-//                 in the previous contact table, contact model
+//   k_bin         wrap positions (modifyParticlePosition), cell index; on a
+//                 list-rebuild step also fills fixed-capacity cell bins and
+//                 the two ghost rows of the Lees-Edwards boundary
+//   k_vl_build    (rebuild steps only) 3x3 stencil of the cell grid with the
+//                 reference's candidate filters; keeps pairs whose bounding
+//                 circles come within a skin; block-compacted pair list
+//   k_pairs       one lane per listed pair: the reference's candidate rule
+//                 and image offset from this step's cells, bounding-circle
+//                 test, ZPM / circle geometry, history, contact model
 //                 (BaseParticle::calculateContactForceAndTorque)
+// The pair list is rebuilt when 2 x (largest displacement of a particle
+// since the last rebuild) + (change of the LE shift) reaches the skin, so
+// every pair that can pass the bounding-circle test is listed.
 //   k_gather      per-particle force/torque sums in a fixed order
 //                 (collectForceAndTorque)
 //   k_integrate   background-velocity removal, integrator, clumps, LE shift
@@ -33,16 +37,23 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxCand = 200;      // src/BaseWorld.h:11
-constexpr int kMaxPairs = 32;      // bounding-circle pairs j>i per particle
 constexpr int kInCap = 32;         // contacts per particle held by partners
 constexpr int kImgShift = 30;
+// The pair/contact table is split into kSeg segments with one bump counter
+// each (kCtrStride ints apart, separate cache lines): thousands of
+// workgroups allocating rows never queue on a single atomic address.
+constexpr int kSeg = 64;
+constexpr int kCtrStride = 32;
 constexpr int kImgStale = 3;
 
 struct Pack {
     double xmin, xmax, ymin, ymax, xlen, ylen, lenx, leny, hx, hy;
     double dt, kn, kt, mu, factor, gdamp, rate;
     double shift, shift_vel, strain;
+    double skin, shift_acc;       // pair-list skin; |LE shift change| since rebuild
+    double vlenx, vleny;          // list grid: cells >= 2 r_max + skin
     int nx, ny, bin_base, world, p_begin, p_end, newton_cap, strict;
+    int vnx, vny;                 // list grid size; bins hold (vny+2)*vnx cells
 };
 
 struct Parts {
@@ -50,24 +61,31 @@ struct Parts {
     const double *r, *ar, *m, *im, *iin;
     double *cbx, *cby;
     const int* up;
+    const int* eid;               // external (caller) index of each particle
     const int8_t* kind;
     const int8_t* flags;
     const int* pack;
     double *fx, *fy, *tq;
     int* ncont;
     int *bin_of, *gbin_of;
+    int *rc_of, *grc_of;          // reference grid: (bin row << 16) | column
+    int* vrc_of;                  // list grid cell of the last rebuild
     int* in_cnt;
     int2* in_ref;
+    double *dsx, *dsy;            // displacement since the list rebuild (a
+                                  // path-length bound for clump subs in dsx)
     int n;
 };
 
 struct Table {
     int *pr_i, *pr_j, *ct_j;
+    int* ej;                      // external index of the partner (ordering key)
     double *xix, *xiy, *nx, *ny, *c;
     double *fnx, *fny, *fsx, *fsy, *t1, *t2, *bx, *by, *px, *py;
     int *row_start, *row_cnt;
-    int* counter;
-    int cap;
+    int* counter;                 // kSeg counters, kCtrStride apart
+    int cap;                      // total entries
+    int seg_cap;                  // entries per segment
 };
 
 struct Bins {
@@ -77,8 +95,8 @@ struct Bins {
     int nbins;
 };
 
-struct Counters {
-    unsigned long long cand, coarse, newton, rescues, fallbacks;
+struct Counters {                  // kSeg copies, 128 B apart
+    unsigned long long cand, coarse, newton, rescues, fallbacks, builds, pad[10];
 };
 
 __device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bit) {
@@ -93,6 +111,10 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 __device__ __forceinline__ void count_add(unsigned long long* dst, unsigned long long v) {
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+__device__ __forceinline__ Counters* counter_slot(Counters* c) {
+    return c + (blockIdx.x % kSeg);
 }
 
 // Image offset of P2 for a pair found in bin row `img`:
@@ -121,15 +143,22 @@ __device__ __forceinline__ void image_offset(const Pack& K, int img, double xi, 
 }
 
 // ---------------------------------------------------------------------------
+// Step start: modifyParticlePosition (LE src/LeesEdwardsBoundaryWorld.cpp:304-333,
+// periodic src/PeriodicBoundaryWorld.cpp:129-142) and the cell of every
+// particle (src/PeriodicBoundaryWorld.cpp:32-33).  On a rebuild step the
+// cell bins and the two ghost rows are filled; otherwise the pair list
+// (rows and segment counters) carries over to the new table.
 __global__ void __launch_bounds__(kBlock)
-k_bin(Parts P, const Pack* __restrict__ packs, Bins B, uint32_t* err) {
+k_bin(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
+      const int* __restrict__ rebuild, int* rebuild_next, uint32_t* err) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p == 0) *rebuild_next = 0;      // set again by k_integrate when needed
     if (p >= P.n || *err) return;
+    const bool rb = *rebuild != 0;
+    if (!rb && p < kSeg) new_t.counter[p * kCtrStride] = old_t.counter[p * kCtrStride];
     const Pack K = packs[P.pack[p]];
     double x = P.x[p], y = P.y[p];
     const double x0 = x, y0 = y;
-    // modifyParticlePosition: LE src/LeesEdwardsBoundaryWorld.cpp:304-333,
-    // periodic src/PeriodicBoundaryWorld.cpp:129-142
     if (K.world == WORLD_LE) {
         if (y < K.ymin) { y += K.ylen; x += K.shift; }
         else if (y > K.ymax) { y += -K.ylen; x += -K.shift; }
@@ -141,23 +170,19 @@ k_bin(Parts P, const Pack* __restrict__ packs, Bins B, uint32_t* err) {
     else if (x > K.xmax) x += -K.xlen;
     if (x != x0 || y != y0) { P.x[p] = x; P.y[p] = y; }
     P.in_cnt[p] = 0;
-    // cell, src/PeriodicBoundaryWorld.cpp:32-33
     int ix = (int)floor((x - K.xmin) / K.lenx);
     int iy = (int)floor((y - K.ymin) / K.leny);
     ix = min(max(ix, 0), K.nx - 1);
     iy = min(max(iy, 0), K.ny - 1);
-    const int bin = K.bin_base + (iy + 1) * K.nx + ix;
-    int rank = atomicAdd(&B.count[bin], 1);
-    if (rank < B.cap) B.list[(size_t)bin * B.cap + rank] = p;
-    else raise_err(err, ERR_BIN_OVERFLOW);
-    P.bin_of[p] = bin;
-    int gbin = -1;
+    P.rc_of[p] = ((iy + 1) << 16) | ix;
+    int grc = -1;
+    int gx = ix;
+    const int grow = iy == 0 ? K.ny + 1 : 0;
     if (iy == 0 || iy == K.ny - 1) {
         // ghost rows: bottom row at x+S feeds the row above the grid, top row
         // at x-S the row below (src/LeesEdwardsBoundaryWorld.cpp:75-88); for
         // the periodic world the unshifted copy reproduces the y wrap of the
         // stencil (src/PeriodicBoundaryWorld.cpp:64-65)
-        int gx = ix;
         if (K.world == WORLD_LE) {
             double t = iy == 0 ? x + K.shift : x - K.shift;
             if (t < K.xmin) t += K.xlen;
@@ -165,94 +190,136 @@ k_bin(Parts P, const Pack* __restrict__ packs, Bins B, uint32_t* err) {
             gx = (int)floor((t - K.xmin) / K.lenx);
             gx = min(max(gx, 0), K.nx - 1);
         }
-        const int row = iy == 0 ? K.ny + 1 : 0;
-        gbin = K.bin_base + row * K.nx + gx;
+        grc = (grow << 16) | gx;
+    }
+    P.grc_of[p] = grc;
+    if (!rb) {
+        new_t.row_start[p] = old_t.row_start[p];
+        new_t.row_cnt[p] = old_t.row_cnt[p];
+        return;
+    }
+    // rebuild step: bins of the list grid, with its own ghost rows
+    int vx_ = (int)floor((x - K.xmin) / K.vlenx);
+    int vy_ = (int)floor((y - K.ymin) / K.vleny);
+    vx_ = min(max(vx_, 0), K.vnx - 1);
+    vy_ = min(max(vy_, 0), K.vny - 1);
+    P.vrc_of[p] = ((vy_ + 1) << 16) | vx_;
+    const int bin = K.bin_base + (vy_ + 1) * K.vnx + vx_;
+    int rank = atomicAdd(&B.count[bin], 1);
+    if (rank < B.cap) B.list[(size_t)bin * B.cap + rank] = p;
+    else raise_err(err, ERR_BIN_OVERFLOW);
+    if (vy_ == 0 || vy_ == K.vny - 1) {
+        int vgx = vx_;
+        if (K.world == WORLD_LE) {
+            double t = vy_ == 0 ? x + K.shift : x - K.shift;
+            if (t < K.xmin) t += K.xlen;
+            else if (t > K.xmax) t -= K.xlen;
+            vgx = (int)floor((t - K.xmin) / K.vlenx);
+            vgx = min(max(vgx, 0), K.vnx - 1);
+        }
+        const int vrow = vy_ == 0 ? K.vny + 1 : 0;
+        const int gbin = K.bin_base + vrow * K.vnx + vgx;
         rank = atomicAdd(&B.count[gbin], 1);
         if (rank < B.cap) B.list[(size_t)gbin * B.cap + rank] = p;
         else raise_err(err, ERR_BIN_OVERFLOW);
     }
-    P.gbin_of[p] = gbin;
 }
 
-// Is P2 = j in P1 = i's 3x3 stencil this step (same filters as k_candidates)?
-__device__ __forceinline__ bool in_stencil(const Pack& K, int bin_i, int bin_j) {
-    const int li = bin_i - K.bin_base, lj = bin_j - K.bin_base;
-    const int ri = li / K.nx, ci = li - ri * K.nx;
-    const int rj = lj / K.nx, cj = lj - rj * K.nx;
+// Is (row, col) in the 3x3 window around (row_i, col_i), columns periodic?
+__device__ __forceinline__ bool in_window(int nx, int rc_i, int rc_j) {
+    const int ri = rc_i >> 16, ci = rc_i & 0xffff;
+    const int rj = rc_j >> 16, cj = rc_j & 0xffff;
     if (rj < ri - 1 || rj > ri + 1) return false;
     int dc = cj - ci;
-    if (dc == K.nx - 1) dc = -1;
-    else if (dc == -(K.nx - 1)) dc = 1;
+    if (dc == nx - 1) dc = -1;
+    else if (dc == -(nx - 1)) dc = 1;
     return dc >= -1 && dc <= 1;
 }
 
-__device__ __forceinline__ bool is_candidate(const Parts& P, const Pack& K, int i, int j) {
+// The reference's candidate rule for P1 = i, P2 = j this step
+// (src/PeriodicBoundaryWorld.cpp:47-84, src/LeesEdwardsBoundaryWorld.cpp:128-204):
+// -1 when j is not in i's candidate lists, otherwise the list it is in:
+// 0 the normal list (minimum image), 1 the LE list of the ghost row above
+// the grid, 2 the LE list of the ghost row below.
+__device__ __forceinline__ int candidate_list(const Parts& P, const Pack& K, int i, int j) {
     if (K.world == WORLD_PERIODIC &&
-        ((P.flags[i] & SKIP_CONTACT_DETECTION) || (P.flags[j] & SKIP_CONTACT_DETECTION)))
-        return false;
-    if (i >= j || P.up[i] == P.up[j]) return false;
-    const int bi = P.bin_of[i];
-    if (in_stencil(K, bi, P.bin_of[j])) return true;
-    const int g = P.gbin_of[j];
-    return g >= 0 && in_stencil(K, bi, g);
+        ((P.flags[i] | P.flags[j]) & SKIP_CONTACT_DETECTION))
+        return -1;
+    if (P.eid[i] >= P.eid[j] || P.up[i] == P.up[j]) return -1;
+    const int ri = P.rc_of[i];
+    if (in_window(K.nx, ri, P.rc_of[j])) return 0;
+    const int g = P.grc_of[j];
+    if (g >= 0 && in_window(K.nx, ri, g)) {
+        if (K.world == WORLD_PERIODIC) return 0;
+        return (g >> 16) == 0 ? 2 : 1;
+    }
+    return -1;
 }
 
+// Pair-list rebuild: one thread per particle i walks the 3x3 stencil of the
+// cell bins (src/PeriodicBoundaryWorld.cpp:58-81, Lees-Edwards ghost rows
+// src/LeesEdwardsBoundaryWorld.cpp:141-199) with the reference's filters and
+// keeps pairs whose bounding circles are closer than the skin; the
+// reference's per-candidate checks (>= 200 candidates, unlike shapes) run
+// here.  History pairs of i missing from the new list are appended so that
+// the pair kernel can decide between erasing and carrying them forward.
+constexpr int kMaxList = 48;       // listed pairs j>i per particle
+
 __global__ void __launch_bounds__(kBlock)
-k_candidates(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
-             uint32_t* err, Counters* cnt) {
-    __shared__ int s_pairs[kMaxPairs][kBlock];
+k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
+           const int* __restrict__ rebuild, uint32_t* err, Counters* cnt) {
+    __shared__ int s_list[kMaxList][kBlock];
     __shared__ int s_scan[kBlock];
     __shared__ int s_base;
+    if (*rebuild == 0) return;
     const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) atomicAdd(&cnt->builds, 1ULL);
     const int i = blockIdx.x * kBlock + tid;
     const bool active = i < P.n && *err == 0;
-    int npair = 0, nstale = 0;
+    int nl = 0, nextra = 0;
     unsigned long long ncand_total = 0;
     if (active) {
         const Pack K = packs[P.pack[i]];
+        P.dsx[i] = 0.0;
+        P.dsy[i] = 0.0;
+        if (i == K.p_begin) packs[P.pack[i]].shift_acc = 0.0;
         const bool skip_i = K.world == WORLD_PERIODIC && (P.flags[i] & SKIP_CONTACT_DETECTION);
         if (!skip_i) {
-            const double xi = P.x[i], yi = P.y[i], ri = P.r[i];
-            const int upi = P.up[i], ki = P.kind[i];
-            const int li = P.bin_of[i] - K.bin_base;
-            const int row_i = li / K.nx, col_i = li - row_i * K.nx;
+            const double xi = P.x[i], yi = P.y[i], ri = P.r[i] + K.skin;
+            const int upi = P.up[i], ki = P.kind[i], eidi = P.eid[i];
+            const int rci = P.vrc_of[i];
+            const int row_i = rci >> 16, col_i = rci & 0xffff;
             int ncand[3] = {0, 0, 0};
             for (int jx0 = -1; jx0 <= 1; ++jx0) {
                 int jx = col_i + jx0;
-                if (jx == -1) jx = K.nx - 1;
-                else if (jx == K.nx) jx = 0;
+                if (jx == -1) jx = K.vnx - 1;
+                else if (jx == K.vnx) jx = 0;
                 for (int jy0 = -1; jy0 <= 1; ++jy0) {
                     const int row = row_i + jy0;
                     int img = 0;
-                    if (K.world == WORLD_LE) img = row == 0 ? 2 : (row == K.ny + 1 ? 1 : 0);
-                    const int b = K.bin_base + row * K.nx + jx;
+                    if (K.world == WORLD_LE) img = row == 0 ? 2 : (row == K.vny + 1 ? 1 : 0);
+                    const int b = K.bin_base + row * K.vnx + jx;
                     const int nb = min(B.count[b], B.cap);
                     const int* lst = B.list + (size_t)b * B.cap;
                     for (int t = 0; t < nb; ++t) {
                         const int j = lst[t];
-                        if (i >= j) continue;
+                        if (eidi >= P.eid[j]) continue;
                         if (K.world == WORLD_PERIODIC && (P.flags[j] & SKIP_CONTACT_DETECTION)) continue;
                         if (upi == P.up[j]) continue;
-                        ncand[img]++;
-                        if (ki == KIND_CLUMP) continue;          // never in contact
-                        const int kj = P.kind[j];
-                        if (kj != ki) { raise_err(err, ERR_UNSUPPORTED); continue; }
+                        // the reference's own candidate lists (its exit checks)
+                        const int cl = candidate_list(P, K, i, j);
+                        if (cl >= 0) {
+                            ncand[cl]++;
+                            if (ki != KIND_CLUMP && P.kind[j] != ki) raise_err(err, ERR_UNSUPPORTED);
+                        }
+                        if (ki == KIND_CLUMP || P.kind[j] != ki) continue;
                         const double xj = P.x[j], yj = P.y[j];
                         double ox, oy, ovx;
                         image_offset(K, img, xi, yi, xj, yj, ox, oy, ovx);
-                        bool pass;
-                        if (ki == KIND_ELLIPSE) {
-                            // takeCoarseDetection, src/BaseParticle.h:131-132
-                            const double qx = xi - (xj + ox), qy = yi - (yj + oy);
-                            pass = (ri + P.r[j]) > sqrt(qx * qx + qy * qy);
-                        } else {
-                            // circles: the contact test itself (src/CircularParticle.cpp:73-80)
-                            const double bx = (xj + ox) - xi, by = (yj + oy) - yi;
-                            pass = !((ri + P.r[j]) - sqrt(bx * bx + by * by) < 0.0);
-                        }
-                        if (pass) {
-                            if (npair < kMaxPairs) s_pairs[npair][tid] = j | (img << kImgShift);
-                            ++npair;
+                        const double qx = xi - (xj + ox), qy = yi - (yj + oy);
+                        if ((ri + P.r[j]) > sqrt(qx * qx + qy * qy)) {
+                            if (nl < kMaxList) s_list[nl][tid] = j;
+                            ++nl;
                         }
                     }
                 }
@@ -261,20 +328,18 @@ k_candidates(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table
                 raise_err(err, ERR_CAND_OVERFLOW);
             ncand_total = (unsigned long long)(ncand[0] + ncand[1] + ncand[2]);
         }
-        if (npair > kMaxPairs) { raise_err(err, ERR_ROW_OVERFLOW); npair = kMaxPairs; }
-        // history entries of i whose pair is no longer a candidate stay
-        // untouched in the reference and keep contributing force
-        // (src/BaseWorld.cpp:34-46)
+        if (nl > kMaxList) { raise_err(err, ERR_ROW_OVERFLOW); nl = kMaxList; }
         const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
         for (int e = os; e < os + oc; ++e) {
             const int j = old_t.ct_j[e];
-            if (j >= 0 && !is_candidate(P, K, i, j)) ++nstale;
+            if (j < 0) continue;
+            bool listed = false;
+            for (int k = 0; k < nl; ++k) listed |= s_list[k][tid] == j;
+            if (!listed) ++nextra;
         }
     }
-    count_add(&cnt->cand, ncand_total);
-    count_add(&cnt->coarse, (unsigned long long)npair);
-    // block exclusive scan of entries per particle
-    const int total = npair + nstale;
+    count_add(&counter_slot(cnt)->cand, ncand_total);
+    const int total = nl + nextra;
     s_scan[tid] = total;
     __syncthreads();
     for (int o = 1; o < kBlock; o <<= 1) {
@@ -285,10 +350,13 @@ k_candidates(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table
     }
     if (tid == kBlock - 1) {
         const int block_total = s_scan[tid];
-        int base = block_total ? atomicAdd(new_t.counter, block_total) : 0;
-        if ((long long)base + block_total > new_t.cap) {
+        const int seg = blockIdx.x % kSeg;
+        int base = block_total ? atomicAdd(new_t.counter + seg * kCtrStride, block_total) : 0;
+        if ((long long)base + block_total > new_t.seg_cap) {
             raise_err(err, ERR_PAIR_OVERFLOW);
             base = -1;
+        } else {
+            base += seg * new_t.seg_cap;
         }
         s_base = base;
     }
@@ -299,81 +367,144 @@ k_candidates(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table
     const int s0 = base + s_scan[tid] - total;
     new_t.row_start[i] = s0;
     new_t.row_cnt[i] = total;
-    for (int k = 0; k < npair; ++k) {
+    for (int k = 0; k < nl; ++k) {
         new_t.pr_i[s0 + k] = i;
-        new_t.pr_j[s0 + k] = s_pairs[k][tid];
+        new_t.pr_j[s0 + k] = s_list[k][tid];
     }
-    if (nstale) {
-        const Pack K = packs[P.pack[i]];
-        int k = npair;
+    if (nextra) {
+        int k = nl;
         const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
         for (int e = os; e < os + oc; ++e) {
             const int j = old_t.ct_j[e];
-            if (j < 0 || is_candidate(P, K, i, j)) continue;
-            const int d = s0 + k++;
-            new_t.pr_i[d] = i;
-            new_t.pr_j[d] = j | (kImgStale << kImgShift);
-            new_t.ct_j[d] = j;
-            new_t.xix[d] = old_t.xix[e]; new_t.xiy[d] = old_t.xiy[e];
-            new_t.nx[d] = old_t.nx[e];   new_t.ny[d] = old_t.ny[e];
-            new_t.c[d] = old_t.c[e];
-            new_t.fnx[d] = old_t.fnx[e]; new_t.fny[d] = old_t.fny[e];
-            new_t.fsx[d] = old_t.fsx[e]; new_t.fsy[d] = old_t.fsy[e];
-            new_t.t1[d] = old_t.t1[e];   new_t.t2[d] = old_t.t2[e];
-            new_t.bx[d] = old_t.bx[e];   new_t.by[d] = old_t.by[e];
-            new_t.px[d] = old_t.px[e];   new_t.py[d] = old_t.py[e];
-            const int q = atomicAdd(&P.in_cnt[j], 1);
-            if (q < kInCap) P.in_ref[(size_t)j * kInCap + q] = make_int2(i, d);
-            else raise_err(err, ERR_INCOMING_OVERFLOW);
+            if (j < 0) continue;
+            bool listed = false;
+            for (int q = 0; q < nl; ++q) listed |= s_list[q][tid] == j;
+            if (listed) continue;
+            new_t.pr_i[s0 + k] = i;
+            new_t.pr_j[s0 + k] = j;
+            ++k;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
-k_pairs(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, uint32_t* err,
-        Counters* cnt) {
+// One lane per listed pair (i, j): the candidate rule and image of this step,
+// bounding-circle test (src/BaseParticle.h:131-132), contact geometry,
+// history and contact model (src/BaseParticle.cpp:27-98).  A history entry
+// whose pair is no longer a candidate is carried forward untouched
+// (src/BaseWorld.cpp:34-46); a candidate out of contact is erased.
+__global__ void __launch_bounds__(kBlock, 2)
+k_pairs(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t,
+        const int* __restrict__ rebuild, uint32_t* err, Counters* cnt) {
+    __shared__ int s_pre[kSeg + 1];
     if (*err) return;
-    const int total = min(*new_t.counter, new_t.cap);
-    unsigned long long n_newton = 0, n_rescue = 0;
-    for (int e = blockIdx.x * kBlock + threadIdx.x; e < total; e += gridDim.x * kBlock) {
-        const int pj = new_t.pr_j[e];
-        const int img = (int)((unsigned)pj >> kImgShift);
-        const int j = pj & ((1 << kImgShift) - 1);
-        const int i = new_t.pr_i[e];
-        if (img == kImgStale || i < 0 || i >= P.n || j >= P.n) continue;
+    const bool rb = *rebuild != 0;
+    const int* __restrict__ src_i = rb ? new_t.pr_i : old_t.pr_i;
+    const int* __restrict__ src_j = rb ? new_t.pr_j : old_t.pr_j;
+    const int* src_ctr = rb ? new_t.counter : old_t.counter;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int g = 0; g < kSeg; ++g) {
+            s_pre[g] = acc;
+            acc += min(src_ctr[g * kCtrStride], new_t.seg_cap);
+        }
+        s_pre[kSeg] = acc;
+    }
+    __syncthreads();
+    const int total = s_pre[kSeg];
+    unsigned long long n_newton = 0, n_rescue = 0, n_coarse = 0;
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < total; w += gridDim.x * kBlock) {
+        int lo = 0, hi = kSeg;                 // s_pre[lo] <= w < s_pre[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[mid] <= w) lo = mid; else hi = mid;
+        }
+        const int e = lo * new_t.seg_cap + (w - s_pre[lo]);
+        const int i = src_i[e];
+        const int j = src_j[e];
+        if (!rb) { new_t.pr_i[e] = i; new_t.pr_j[e] = j; }
+        if (i < 0 || i >= P.n || j < 0 || j >= P.n) { new_t.ct_j[e] = -1; continue; }
         const Pack K = packs[P.pack[i]];
-        const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
-        double ox, oy, ovx;
-        image_offset(K, img, xi, yi, xj, yj, ox, oy, ovx);
-        const double xj2 = xj + ox, yj2 = yj + oy;
-        const double dx = xj2 - xi, dy = yj2 - yi;
-        Geom g;
-        int hit;
-        if (P.kind[i] == KIND_CIRCLE) {
-            hit = circle_contact(xi, yi, P.r[i], xj2, yj2, P.r[j], g) ? 1 : 0;
+        // history of (i, j): the same slot when the list was not rebuilt
+        int found = -1;
+        if (!rb) {
+            if (old_t.ct_j[e] == j) found = e;
         } else {
-            ZpmSetup S;
-            hit = zpm_setup(P.r[i], P.ar[i], P.th[i], P.r[j], P.ar[j], P.th[j], dx, dy, S);
-            if (hit) {
-                const double qinit = sqrt(S.delta + 1.0);
-                double q;
-                int it = newton_quartic(S.A, S.B, S.C, S.D, S.E, qinit, K.newton_cap, &q);
-                if (it < 0) {
-                    if (K.strict || !quartic_bisect(S.A, S.B, S.C, S.D, S.E, qinit, &q)) {
-                        raise_err(err, ERR_NEWTON_CAP);
-                        new_t.ct_j[e] = -1;
-                        continue;
+            const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
+            for (int t = os; t < os + oc; ++t)
+                if (old_t.ct_j[t] == j) { found = t; break; }
+        }
+        const int lst = candidate_list(P, K, i, j);
+        const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
+        double ox = 0.0, oy = 0.0, ovx = 0.0;
+        Geom g;
+        int hit = 0;
+        if (lst >= 0 && P.kind[i] != KIND_CLUMP && P.kind[j] == P.kind[i]) {
+            image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
+            const double xj2 = xj + ox, yj2 = yj + oy;
+            const double dx = xj2 - xi, dy = yj2 - yi;
+            if (P.kind[i] == KIND_CIRCLE) {
+                ++n_coarse;
+                hit = circle_contact(xi, yi, P.r[i], xj2, yj2, P.r[j], g) ? 1 : 0;
+            } else {
+                const double qx = xi - (xj + ox), qy = yi - (yj + oy);
+                if ((P.r[i] + P.r[j]) > sqrt(qx * qx + qy * qy)) {
+                    ++n_coarse;
+                    // attempt 0: the reference's arithmetic; attempt 1 only
+                    // when that gives no valid answer (non-finite contact
+                    // geometry): eigenvalues ordered by |lambda|
+                    for (int attempt = 0; attempt < 2; ++attempt) {
+                        ZpmSetup S;
+                        hit = zpm_setup(P.r[i], P.ar[i], P.th[i], P.r[j], P.ar[j], P.th[j], dx,
+                                        dy, S, attempt == 1);
+                        if (!hit) break;
+                        const double qinit = sqrt(S.delta + 1.0);
+                        double q;
+                        int it = newton_quartic(S.A, S.B, S.C, S.D, S.E, qinit, K.newton_cap, &q);
+                        if (it < 0) {
+                            if (K.strict || !quartic_bisect(S.A, S.B, S.C, S.D, S.E, qinit, &q)) {
+                                hit = -2;
+                                break;
+                            }
+                            ++n_rescue;
+                            it = K.newton_cap;
+                        }
+                        n_newton += (unsigned long long)it;
+                        hit = zpm_finish(S, q, xi, yi, xj2, yj2, dx, dy, g);
+                        if (hit == 1 && attempt == 0 &&
+                            !(isfinite(g.overlap) && isfinite(g.nx) && isfinite(g.ny) &&
+                              isfinite(g.px) && isfinite(g.py))) {
+                            if (K.strict) { hit = -2; break; }
+                            ++n_rescue;
+                            continue;
+                        }
+                        break;
                     }
-                    ++n_rescue;
-                    it = K.newton_cap;
+                    if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); new_t.ct_j[e] = -1; continue; }
+                    if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); new_t.ct_j[e] = -1; continue; }
                 }
-                n_newton += (unsigned long long)it;
-                hit = zpm_finish(S, q, xi, yi, xj2, yj2, dx, dy, g);
-                if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); new_t.ct_j[e] = -1; continue; }
             }
         }
-        if (!hit) { new_t.ct_j[e] = -1; continue; }
+        if (!hit) {
+            if (found >= 0 && lst < 0) {           // not a candidate: carried forward
+                new_t.ct_j[e] = j;
+                new_t.ej[e] = old_t.ej[found];
+                new_t.xix[e] = old_t.xix[found]; new_t.xiy[e] = old_t.xiy[found];
+                new_t.nx[e] = old_t.nx[found];   new_t.ny[e] = old_t.ny[found];
+                new_t.c[e] = old_t.c[found];
+                new_t.fnx[e] = old_t.fnx[found]; new_t.fny[e] = old_t.fny[found];
+                new_t.fsx[e] = old_t.fsx[found]; new_t.fsy[e] = old_t.fsy[found];
+                new_t.t1[e] = old_t.t1[found];   new_t.t2[e] = old_t.t2[found];
+                new_t.bx[e] = old_t.bx[found];   new_t.by[e] = old_t.by[found];
+                new_t.px[e] = old_t.px[found];   new_t.py[e] = old_t.py[found];
+                const int q = atomicAdd(&P.in_cnt[j], 1);
+                if (q < kInCap) P.in_ref[(size_t)j * kInCap + q] = make_int2(P.eid[i], e);
+                else raise_err(err, ERR_INCOMING_OVERFLOW);
+            } else {
+                new_t.ct_j[e] = -1;
+            }
+            continue;
+        }
         // relative velocity, src/BaseParticle.cpp:11-25; LE velocities carry
         // the background term vx + rate*y (src/LeesEdwardsBoundaryWorld.cpp:508-518)
         const bool le = K.world == WORLD_LE;
@@ -386,13 +517,9 @@ k_pairs(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, uint3
         const double sv = rvx * g.nx + rvy * g.ny;
         const double nrvx = g.nx * sv, nrvy = g.ny * sv;
         const double srvx = rvx - nrvx, srvy = rvy - nrvy;
-        // history, src/BaseParticle.cpp:36-82
-        const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
-        int found = -1;
-        for (int t = os; t < os + oc; ++t)
-            if (old_t.ct_j[t] == j) { found = t; break; }
         double xix, xiy, c;
         if (found < 0) {
+            // new contact, src/BaseParticle.cpp:38-64
             const double vn = rvx * g.nx + rvy * g.ny;
             if (!isnormal(vn)) { xix = srvx * K.dt; xiy = srvy * K.dt; }
             else {
@@ -409,6 +536,7 @@ k_pairs(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, uint3
                 c = sqrt(m0) * K.factor;
             }
         } else {
+            // persisting contact, src/BaseParticle.cpp:67-82
             const double onx = old_t.nx[found], ony = old_t.ny[found];
             const double ox0 = old_t.xix[found], oy0 = old_t.xiy[found];
             const double dot = onx * g.nx + ony * g.ny;
@@ -439,6 +567,7 @@ k_pairs(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, uint3
         // src/ContactInformation.h:62-68
         const double fx = fnx + fsx, fy = fny + fsy;
         new_t.ct_j[e] = j;
+        new_t.ej[e] = P.eid[j];
         new_t.xix[e] = xix; new_t.xiy[e] = xiy;
         new_t.nx[e] = g.nx; new_t.ny[e] = g.ny;
         new_t.c[e] = c;
@@ -450,11 +579,13 @@ k_pairs(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, uint3
         new_t.by[e] = yi - (yj + oy);
         new_t.px[e] = g.px; new_t.py[e] = g.py;
         const int q = atomicAdd(&P.in_cnt[j], 1);
-        if (q < kInCap) P.in_ref[(size_t)j * kInCap + q] = make_int2(i, e);
+        if (q < kInCap) P.in_ref[(size_t)j * kInCap + q] = make_int2(P.eid[i], e);
         else raise_err(err, ERR_INCOMING_OVERFLOW);
     }
-    count_add(&cnt->newton, n_newton);
-    count_add(&cnt->rescues, n_rescue);
+    Counters* cs = counter_slot(cnt);
+    count_add(&cs->newton, n_newton);
+    count_add(&cs->rescues, n_rescue);
+    count_add(&cs->coarse, n_coarse);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,7 +619,7 @@ k_gather(Parts P, Table t, uint32_t* err) {
     for (int k = 0; k < cnt; ++k) {
         int best = 0x7fffffff, be = -1;
         for (int e = s; e < s + cnt; ++e) {
-            const int j = t.ct_j[e];
+            const int j = t.ct_j[e] >= 0 ? t.ej[e] : -1;
             if (j > last && j < best) { best = j; be = e; }
         }
         if (be < 0) break;
@@ -507,10 +638,12 @@ k_gather(Parts P, Table t, uint32_t* err) {
 // update (src/LeesEdwardsBoundaryWorld.cpp:285-296) and clearing of the
 // per-step bins and the next table's counter.
 __global__ void __launch_bounds__(kBlock)
-k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, uint32_t* err) {
+k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
+            int* rebuild_next, uint32_t* err) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
-    for (int b = p; b < B.nbins; b += gridDim.x * kBlock) B.count[b] = 0;
-    if (p == 0) *next_counter = 0;
+    if (*rebuild)
+        for (int b = p; b < B.nbins; b += gridDim.x * kBlock) B.count[b] = 0;
+    if (p < kSeg) next_counter[p * kCtrStride] = 0;
     if (p >= P.n || *err) return;
     const int pk = P.pack[p];
     const Pack K = packs[pk];
@@ -533,10 +666,16 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, uint32_t* err) {
             w = (w * (1.0 - g * dt) + P.tq[p] * P.iin[p] * dt) / (1.0 + g * dt);
             P.w[p] = w;
             P.th[p] += w * dt;
+            P.dsx[p] += vx * dt;
+            P.dsy[p] += vy * dt;
+            // the reference carries NaN on (0/0 normals of coincident
+            // centres); the engine stops and reports it
+            if (!isfinite(P.x[p]) || !isfinite(P.y[p])) raise_err(err, ERR_NONFINITE);
         } else {
             double tfx = 0.0, tfy = 0.0, tt = 0.0;
             int s = p + 1;
-            for (; s < P.n && P.up[s] == p; ++s) {
+            const int ep = P.eid[p];
+            for (; s < P.n && P.up[s] == ep; ++s) {
                 tfx += P.fx[s]; tfy += P.fy[s];
                 tt += P.tq[s];
                 tt += P.cbx[s] * P.fy[s] - P.cby[s] * P.fx[s];
@@ -569,6 +708,11 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, uint32_t* err) {
                 P.cbx[k] = bx; P.cby[k] = by;
                 P.vx[k] = vx + (-w * by);
                 P.vy[k] = vy + w * bx;
+                // path-length bound of the sub's motion: centre + rotation
+                const double hop = sqrt(vx * vx + vy * vy) * dt + fabs(w) * dt * sqrt(bx * bx + by * by);
+                const double d = (P.dsx[k] += hop);
+                if (2.0 * d + K.shift_acc + fabs(K.shift_vel * K.dt) >= K.skin && *rebuild_next == 0)
+                    atomicOr(rebuild_next, 1);
                 P.x[k] = x + bx;
                 P.y[k] = y + by;
                 P.w[k] = w;
@@ -576,6 +720,12 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, uint32_t* err) {
             }
         }
     }
+    // pair-list validity: a pair can close the skin only if the two
+    // particles together travelled it, plus the LE shift change
+    const double dshift = K.world == WORLD_LE ? fabs(K.rate * K.ylen * K.dt) : 0.0;
+    const double ddx = P.dsx[p], ddy = P.dsy[p];
+    if (2.0 * sqrt(ddx * ddx + ddy * ddy) + K.shift_acc + dshift >= K.skin && *rebuild_next == 0)
+        atomicOr(rebuild_next, 1);
     if (p == K.p_begin && K.world == WORLD_LE) {
         Pack& W = packs[pk];
         W.shift_vel = K.rate * K.ylen;
@@ -584,6 +734,7 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, uint32_t* err) {
         if (sh > K.xmax) sh -= K.xlen;
         else if (sh < K.xmin) sh += K.xlen;
         W.shift = sh;
+        W.shift_acc = K.shift_acc + dshift;
     }
 }
 
@@ -598,15 +749,16 @@ k_stress(Parts P, Table t, double* out) {
     if (!(P.flags[p] & SKIP_MACRO_SUM)) {
         const bool clump = P.kind[p] == KIND_CLUMP;
         const int s_end = clump ? P.n : p + 1;
+        const int ep = P.eid[p];
         for (int s = clump ? p + 1 : p; s < s_end; ++s) {
-            if (clump && P.up[s] != p) break;
+            if (clump && P.up[s] != ep) break;
             const double ox = clump ? P.cbx[s] : 0.0, oy = clump ? P.cby[s] : 0.0;
             const int rs = t.row_start[s], cnt = t.row_cnt[s];
             int last = -1;
             for (int k = 0; k < cnt; ++k) {
                 int best = 0x7fffffff, be = -1;
                 for (int e = rs; e < rs + cnt; ++e) {
-                    const int j = t.ct_j[e];
+                    const int j = t.ct_j[e] >= 0 ? t.ej[e] : -1;
                     if (j > last && j < best) { best = j; be = e; }
                 }
                 if (be < 0) break;
@@ -624,14 +776,16 @@ k_stress(Parts P, Table t, double* out) {
 
 // history restart: one contact per thread into the old table's rows
 __global__ void k_load_contacts(Table t, int nc, const int* ci, const int* cj,
-                                const double* xix, const double* xiy, const double* nx,
-                                const double* ny, const double* c, const int* slot) {
+                                const int* ej, const double* xix, const double* xiy,
+                                const double* nx, const double* ny, const double* c,
+                                const int* slot) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nc) return;
     const int e = slot[k];
     t.pr_i[e] = ci[k];
     t.pr_j[e] = cj[k] | (kImgStale << kImgShift);
     t.ct_j[e] = cj[k];
+    t.ej[e] = ej[k];
     t.xix[e] = xix[k]; t.xiy[e] = xiy[k]; t.nx[e] = nx[k]; t.ny[e] = ny[k]; t.c[e] = c[k];
     t.fnx[e] = 0; t.fny[e] = 0; t.fsx[e] = 0; t.fsy[e] = 0; t.t1[e] = 0; t.t2[e] = 0;
     t.bx[e] = 0; t.by[e] = 0; t.px[e] = 0; t.py[e] = 0;
@@ -661,8 +815,8 @@ struct xdem_engine {
     int n = 0;
     int npacks = 0;
     std::vector<int> pack_start;
-    std::vector<int8_t> kind_h;
-    std::vector<int> up_h;
+    std::vector<int> perm;        // internal index -> external index
+    std::vector<int> inv;         // external index -> internal index
     // device memory
     std::vector<void*> allocs;
     Parts P{};
@@ -672,6 +826,7 @@ struct xdem_engine {
     uint32_t* err_d = nullptr;
     Counters* cnt_d = nullptr;
     double* stress_d = nullptr;
+    int* rebuild_d = nullptr;     // two alternating pair-list rebuild flags
     long long steps = 0;
     bool have_particles = false;
     bool timing = false;
@@ -680,6 +835,10 @@ struct xdem_engine {
     double timed_ms = 0.0;
     long long timed_launches = 0;
     int pair_grid = 0;
+    // two-step hipGraphs, one per step parity (tables and flags ping-pong)
+    bool use_graphs = true;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -701,19 +860,47 @@ int dalloc(xdem_engine* E, T** p, size_t count) {
     return XDEM_OK;
 }
 
+// device array in internal order -> host array in external order
+template <class T>
+int fetch_ext(xdem_engine* E, T* dst, const T* src) {
+    if (!dst) return XDEM_OK;
+    std::vector<T> tmp(E->n);
+    HIPCHK(hipMemcpy(tmp.data(), src, sizeof(T) * (size_t)E->n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < E->n; ++k) dst[E->perm[k]] = tmp[k];
+    return XDEM_OK;
+}
+
 int map_error_bits(xdem_engine* E, uint32_t bits) {
     if (!bits) return XDEM_OK;
     std::string m = "step failed:";
     int code = XDEM_E_CAPACITY;
     if (bits & ERR_UNSUPPORTED) { m += " unlike particle shapes in a candidate pair (reference exit(-1), src/EllipticParticle.cpp:30);"; code = XDEM_E_UNSUPPORTED_PAIR; }
     if (bits & ERR_ZERO_NORMAL) { m += " zero-length contact normal (reference exit(-1), src/MathVector.h:187);"; code = XDEM_E_ZERO_NORMAL; }
-    if (bits & ERR_NEWTON_CAP) { m += " quartic Newton reached its cap without a bracketed rescue;"; code = XDEM_E_NEWTON_CAP; }
+    if (bits & ERR_NEWTON_CAP) { m += " the reference's arithmetic gives no valid contact for a pair (Newton cap without a bracketed root, or non-finite geometry) and strict mode is on or the rescue failed;"; code = XDEM_E_NEWTON_CAP; }
     if (bits & ERR_CAND_OVERFLOW) { m += " a particle has >= 200 candidates (reference exit(-2));"; code = XDEM_E_CAPACITY; }
     if (bits & ERR_BIN_OVERFLOW) { m += " cell bin capacity exceeded;"; code = XDEM_E_CAPACITY; }
     if (bits & ERR_PAIR_OVERFLOW) { m += " contact table capacity exceeded;"; code = XDEM_E_CAPACITY; }
     if (bits & ERR_INCOMING_OVERFLOW) { m += " more than 32 contacts held by partners of one particle;"; code = XDEM_E_CAPACITY; }
-    if (bits & ERR_ROW_OVERFLOW) { m += " more than 32 bounding-circle pairs for one particle;"; code = XDEM_E_CAPACITY; }
+    if (bits & ERR_NONFINITE) { m += " a particle position became non-finite (coincident centres give 0/0 normals);"; code = XDEM_E_NONFINITE; }
+    if (bits & ERR_ROW_OVERFLOW) { m += " more than 512 bounding-circle pairs in one cell;"; code = XDEM_E_CAPACITY; }
     return fail(E, code, m);
+}
+
+// entries in use per segment (the restart table of xdem_set_contacts keeps
+// its rows in segment 0 and no counters)
+int segment_counts(xdem_engine* E, const Table& t, std::vector<int>& used) {
+    std::vector<int> c((size_t)kSeg * kCtrStride);
+    HIPCHK(hipMemcpy(c.data(), t.counter, sizeof(int) * c.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < kSeg; ++s) used[s] = std::min(c[(size_t)s * kCtrStride], t.seg_cap);
+    if (E->steps == 0) {
+        std::vector<int> cnt(E->n);
+        HIPCHK(hipMemcpy(cnt.data(), t.row_cnt, sizeof(int) * E->n, hipMemcpyDeviceToHost));
+        long long tot = 0;
+        for (int v : cnt) tot += v;
+        used.assign(kSeg, 0);
+        used[0] = (int)tot;
+    }
+    return XDEM_OK;
 }
 
 int check_error(xdem_engine* E) {
@@ -725,24 +912,46 @@ int check_error(xdem_engine* E) {
 
 int alloc_table(xdem_engine* E, Table& t, int cap, int n) {
     int rc;
-    t.cap = cap;
+    t.seg_cap = cap / kSeg;
+    t.cap = t.seg_cap * kSeg;
+    cap = t.cap;
     if ((rc = dalloc(E, &t.pr_i, cap)) || (rc = dalloc(E, &t.pr_j, cap)) ||
-        (rc = dalloc(E, &t.ct_j, cap)))
+        (rc = dalloc(E, &t.ct_j, cap)) || (rc = dalloc(E, &t.ej, cap)))
         return rc;
     double** d[] = {&t.xix, &t.xiy, &t.nx, &t.ny, &t.c, &t.fnx, &t.fny, &t.fsx,
                     &t.fsy, &t.t1, &t.t2, &t.bx, &t.by, &t.px, &t.py};
     for (double** q : d)
         if ((rc = dalloc(E, q, cap))) return rc;
     if ((rc = dalloc(E, &t.row_start, n)) || (rc = dalloc(E, &t.row_cnt, n)) ||
-        (rc = dalloc(E, &t.counter, 1)))
+        (rc = dalloc(E, &t.counter, kSeg * kCtrStride)))
         return rc;
     HIPCHK(hipMemset(t.row_start, 0, sizeof(int) * (size_t)n));
     HIPCHK(hipMemset(t.row_cnt, 0, sizeof(int) * (size_t)n));
-    HIPCHK(hipMemset(t.counter, 0, sizeof(int)));
+    HIPCHK(hipMemset(t.counter, 0, sizeof(int) * kSeg * kCtrStride));
     return XDEM_OK;
 }
 
 int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
+
+uint64_t morton2(uint32_t x, uint32_t y) {
+    auto spread = [](uint64_t v) {
+        v &= 0xffffffffULL;
+        v = (v | (v << 16)) & 0x0000ffff0000ffffULL;
+        v = (v | (v << 8)) & 0x00ff00ff00ff00ffULL;
+        v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0fULL;
+        v = (v | (v << 2)) & 0x3333333333333333ULL;
+        v = (v | (v << 1)) & 0x5555555555555555ULL;
+        return v;
+    };
+    return spread(x) | (spread(y) << 1);
+}
+
+template <class T>
+std::vector<T> permuted(const T* src, const std::vector<int>& perm) {
+    std::vector<T> out(perm.size());
+    for (size_t k = 0; k < perm.size(); ++k) out[k] = src[perm[k]];
+    return out;
+}
 
 }  // namespace
 
@@ -780,6 +989,10 @@ int xdem_create(xdem_engine** out, int num_packings, const xdem_config* cfgs) {
 void xdem_destroy(xdem_engine* E) {
     if (!E) return;
     if (E->stream) hipStreamSynchronize(E->stream);
+    for (int k = 0; k < 2; ++k) {
+        if (E->graph_exec[k]) hipGraphExecDestroy(E->graph_exec[k]);
+        if (E->graph[k]) hipGraphDestroy(E->graph[k]);
+    }
     for (void* p : E->allocs) hipFree(p);
     for (hipEvent_t ev : E->ev_pool) hipEventDestroy(ev);
     if (E->stream) hipStreamDestroy(E->stream);
@@ -846,13 +1059,49 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
             return fail(E, XDEM_E_GRID, "packing " + std::to_string(k) + ": cell grid smaller than 3x3");
         K.lenx = K.xlen / K.nx;
         K.leny = K.ylen / K.ny;
+        // pair list: pairs closer than r_i + r_j + skin, found on a list grid
+        // whose cells (>= 2 r_max + skin) hold every such pair in a 3x3 stencil
+        K.skin = c.skin > 0 ? c.skin : 0.2 * rmax;
+        K.shift_acc = 0.0;
+        const double vcell = 2.0 * rmax + K.skin;
+        K.vnx = std::max(3, (int)std::floor(K.xlen / vcell));
+        K.vny = std::max(3, (int)std::floor(K.ylen / vcell));
+        K.vlenx = K.xlen / K.vnx;
+        K.vleny = K.ylen / K.vny;
+        if (K.vlenx < vcell || K.vleny < vcell) K.skin = std::max(0.0, std::min(K.vlenx, K.vleny) - 2.0 * rmax);
         K.hx = K.xlen / 2.0; K.hy = K.ylen / 2.0;
         K.bin_base = bins_total;
-        bins_total += (K.ny + 2) * K.nx;
-        if (amin < 1e300) max_ratio = std::max(max_ratio, K.lenx * K.leny / amin);
+        bins_total += (K.vny + 2) * K.vnx;
+        if (amin < 1e300) max_ratio = std::max(max_ratio, K.vlenx * K.vleny / amin);
     }
     int bin_cap = 32;
     while (bin_cap < 2.5 * max_ratio + 8 && bin_cap < 1024) bin_cap *= 2;
+    // Particles are stored in Morton order of their cell (a clump with its
+    // sub-particles as one unit), so that neighbours are neighbours in
+    // memory; external indices keep every ID-dependent rule of the reference.
+    {
+        std::vector<std::pair<uint64_t, long long>> key(n);
+        for (int k = 0; k < E->npacks; ++k) {
+            const Pack& K = E->packs_h[k];
+            for (int p = ps[k]; p < ps[k + 1]; ++p) {
+                int lead = p;
+                const int u = up_level[p];
+                if (u != p && u >= ps[k] && u < ps[k + 1] && kind[u] == XDEM_KIND_CLUMP) lead = u;
+                double fx = std::floor((x[lead] - K.xmin) / K.lenx);
+                double fy = std::floor((y[lead] - K.ymin) / K.leny);
+                const int cx = (int)std::min<double>(std::max<double>(fx, 0.0), K.nx - 1);
+                const int cy = (int)std::min<double>(std::max<double>(fy, 0.0), K.ny - 1);
+                key[p] = {((uint64_t)k << 48) | morton2(cx, cy), ((long long)lead << 32) | p};
+            }
+        }
+        std::vector<int> order(n);
+        for (int p = 0; p < n; ++p) order[p] = p;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+        E->perm = order;
+        E->inv.assign(n, 0);
+        for (int k = 0; k < n; ++k) E->inv[order[k]] = k;
+    }
+    const std::vector<int>& perm = E->perm;
     int rc;
     Parts& P = E->P;
     P.n = n;
@@ -860,44 +1109,50 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     for (double** q : st)
         if ((rc = dalloc(E, q, n))) return rc;
     double *r_d, *ar_d, *m_d, *im_d, *iin_d;
-    int *up_d, *pack_d;
+    int *up_d, *pack_d, *eid_d;
     int8_t *kind_d, *flags_d;
     if ((rc = dalloc(E, &r_d, n)) || (rc = dalloc(E, &ar_d, n)) || (rc = dalloc(E, &m_d, n)) ||
         (rc = dalloc(E, &im_d, n)) || (rc = dalloc(E, &iin_d, n)) || (rc = dalloc(E, &up_d, n)) ||
-        (rc = dalloc(E, &pack_d, n)) || (rc = dalloc(E, &kind_d, n)) ||
+        (rc = dalloc(E, &pack_d, n)) || (rc = dalloc(E, &eid_d, n)) || (rc = dalloc(E, &kind_d, n)) ||
         (rc = dalloc(E, &flags_d, n)) || (rc = dalloc(E, &P.ncont, n)) ||
         (rc = dalloc(E, &P.bin_of, n)) || (rc = dalloc(E, &P.gbin_of, n)) ||
+        (rc = dalloc(E, &P.rc_of, n)) || (rc = dalloc(E, &P.grc_of, n)) ||
+        (rc = dalloc(E, &P.vrc_of, n)) ||
+        (rc = dalloc(E, &P.dsx, n)) || (rc = dalloc(E, &P.dsy, n)) ||
         (rc = dalloc(E, &P.in_cnt, n)) || (rc = dalloc(E, &P.in_ref, (size_t)n * kInCap)))
         return rc;
     P.r = r_d; P.ar = ar_d; P.m = m_d; P.im = im_d; P.iin = iin_d;
-    P.up = up_d; P.pack = pack_d; P.kind = kind_d; P.flags = flags_d;
+    P.up = up_d; P.pack = pack_d; P.eid = eid_d; P.kind = kind_d; P.flags = flags_d;
     std::vector<double> im(n), iin(n), cb0(n, 0.0);
     std::vector<int8_t> k8(n), f8(n);
-    for (int p = 0; p < n; ++p) {
-        im[p] = 1.0 / mass[p];
-        iin[p] = 1.0 / inertia[p];
-        k8[p] = (int8_t)kind[p];
-        f8[p] = (int8_t)flags[p];
+    for (int k = 0; k < n; ++k) {
+        const int p = perm[k];
         if (kind[p] < 0 || kind[p] > 2) return fail(E, XDEM_E_INVALID, "bad particle kind");
+        im[k] = 1.0 / mass[p];
+        iin[k] = 1.0 / inertia[p];
+        k8[k] = (int8_t)kind[p];
+        f8[k] = (int8_t)flags[p];
     }
-    E->kind_h = k8;
-    E->up_h.assign(up_level, up_level + n);
     const size_t db = sizeof(double) * (size_t)n, ib = sizeof(int) * (size_t)n;
-    HIPCHK(hipMemcpy(P.x, x, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.y, y, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.th, theta, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.vx, vx, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.vy, vy, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.w, omega, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.cbx, clump_bx ? clump_bx : cb0.data(), db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.cby, clump_by ? clump_by : cb0.data(), db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(r_d, outer_r, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ar_d, aspect, db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(m_d, mass, db, hipMemcpyHostToDevice));
+    auto up_f64 = [&](double* dst, const double* src) -> int {
+        std::vector<double> v = src ? permuted(src, perm) : cb0;
+        HIPCHK(hipMemcpy(dst, v.data(), db, hipMemcpyHostToDevice));
+        return XDEM_OK;
+    };
+    if ((rc = up_f64(P.x, x)) || (rc = up_f64(P.y, y)) || (rc = up_f64(P.th, theta)) ||
+        (rc = up_f64(P.vx, vx)) || (rc = up_f64(P.vy, vy)) || (rc = up_f64(P.w, omega)) ||
+        (rc = up_f64(P.cbx, clump_bx)) || (rc = up_f64(P.cby, clump_by)) ||
+        (rc = up_f64(r_d, outer_r)) || (rc = up_f64(ar_d, aspect)) || (rc = up_f64(m_d, mass)))
+        return rc;
     HIPCHK(hipMemcpy(im_d, im.data(), db, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(iin_d, iin.data(), db, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(up_d, up_level, ib, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(pack_d, pack_of.data(), ib, hipMemcpyHostToDevice));
+    {
+        std::vector<int> v = permuted(up_level, perm);
+        HIPCHK(hipMemcpy(up_d, v.data(), ib, hipMemcpyHostToDevice));
+        v = permuted(pack_of.data(), perm);
+        HIPCHK(hipMemcpy(pack_d, v.data(), ib, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(eid_d, perm.data(), ib, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(kind_d, k8.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(flags_d, f8.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(P.fx, 0, db));
@@ -905,6 +1160,8 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     HIPCHK(hipMemset(P.tq, 0, db));
     HIPCHK(hipMemset(P.ncont, 0, ib));
     HIPCHK(hipMemset(P.in_cnt, 0, ib));
+    HIPCHK(hipMemset(P.dsx, 0, db));
+    HIPCHK(hipMemset(P.dsy, 0, db));
     // bins
     E->B.cap = bin_cap;
     E->B.nbins = bins_total;
@@ -912,16 +1169,26 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         (rc = dalloc(E, &E->B.list, (size_t)bins_total * bin_cap)))
         return rc;
     HIPCHK(hipMemset(E->B.count, 0, sizeof(int) * (size_t)bins_total));
+    for (int k = 0; k < E->npacks; ++k) {
+        const Pack& K = E->packs_h[k];
+        if (K.nx > 65535 || K.ny > 65534)
+            return fail(E, XDEM_E_GRID, "cell grid larger than 65535 per side");
+    }
     // contact tables
-    const long long tcap = std::min<long long>(12LL * n + 65536, (1LL << kImgShift) - 1);
+    const long long tcap = std::min<long long>(12LL * n + 64LL * 4096, (1LL << kImgShift) - 1);
     if ((rc = alloc_table(E, E->T[0], (int)tcap, n)) || (rc = alloc_table(E, E->T[1], (int)tcap, n)))
         return rc;
     if ((rc = dalloc(E, &E->packs_d, E->npacks)) || (rc = dalloc(E, &E->err_d, 1)) ||
-        (rc = dalloc(E, &E->cnt_d, 1)) || (rc = dalloc(E, &E->stress_d, (size_t)4 * n)))
+        (rc = dalloc(E, &E->cnt_d, kSeg)) || (rc = dalloc(E, &E->stress_d, (size_t)4 * n)) ||
+        (rc = dalloc(E, &E->rebuild_d, 2)))
         return rc;
     HIPCHK(hipMemcpy(E->packs_d, E->packs_h.data(), sizeof(Pack) * E->npacks, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(E->err_d, 0, sizeof(uint32_t)));
-    HIPCHK(hipMemset(E->cnt_d, 0, sizeof(Counters)));
+    HIPCHK(hipMemset(E->cnt_d, 0, sizeof(Counters) * kSeg));
+    {
+        const int flags0[2] = {1, 0};     // the first step builds the pair list
+        HIPCHK(hipMemcpy(E->rebuild_d, flags0, sizeof flags0, hipMemcpyHostToDevice));
+    }
     E->cur = 0;
     E->have_particles = true;
     return XDEM_OK;
@@ -934,29 +1201,33 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
     if (nc < 0 || (nc > 0 && (!ci || !cj || !xi_x || !xi_y || !n_x || !n_y || !damping)))
         return fail(E, XDEM_E_INVALID, "bad contact arrays");
     // rows of the table that the next step reads as "previous"
-    std::vector<int> order(nc), cnt(E->n, 0), start(E->n, 0), slot(nc);
+    std::vector<int> cnt(E->n, 0), start(E->n, 0), slot(nc), ii(nc), jj(nc);
     for (int k = 0; k < nc; ++k) {
         if (ci[k] < 0 || ci[k] >= E->n || cj[k] <= ci[k] || cj[k] >= E->n)
             return fail(E, XDEM_E_INVALID, "contact indices must satisfy 0 <= i < j < n");
-        cnt[ci[k]]++;
+        ii[k] = E->inv[ci[k]];
+        jj[k] = E->inv[cj[k]];
+        cnt[ii[k]]++;
     }
     int acc = 0;
     for (int p = 0; p < E->n; ++p) { start[p] = acc; acc += cnt[p]; }
-    if (acc > E->T[0].cap) return fail(E, XDEM_E_CAPACITY, "too many contacts");
+    if (acc > E->T[0].seg_cap) return fail(E, XDEM_E_CAPACITY, "too many contacts");
     std::vector<int> fill(E->n, 0);
-    for (int k = 0; k < nc; ++k) slot[k] = start[ci[k]] + fill[ci[k]]++;
+    for (int k = 0; k < nc; ++k) slot[k] = start[ii[k]] + fill[ii[k]]++;
     Table& t = E->T[E->cur];
     int rc;
-    int *ci_d, *cj_d, *slot_d;
+    int *ci_d, *cj_d, *ej_d, *slot_d;
     double *a_d, *b_d, *c_d, *d_d, *e_d;
     const size_t base = E->allocs.size();
     if ((rc = dalloc(E, &ci_d, nc)) || (rc = dalloc(E, &cj_d, nc)) || (rc = dalloc(E, &slot_d, nc)) ||
+        (rc = dalloc(E, &ej_d, nc)) ||
         (rc = dalloc(E, &a_d, nc)) || (rc = dalloc(E, &b_d, nc)) || (rc = dalloc(E, &c_d, nc)) ||
         (rc = dalloc(E, &d_d, nc)) || (rc = dalloc(E, &e_d, nc)))
         return rc;
     if (nc > 0) {
-        HIPCHK(hipMemcpy(ci_d, ci, sizeof(int) * nc, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cj_d, cj, sizeof(int) * nc, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ci_d, ii.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cj_d, jj.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ej_d, cj, sizeof(int) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(slot_d, slot.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(a_d, xi_x, sizeof(double) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b_d, xi_y, sizeof(double) * nc, hipMemcpyHostToDevice));
@@ -964,7 +1235,7 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
         HIPCHK(hipMemcpy(d_d, n_y, sizeof(double) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e_d, damping, sizeof(double) * nc, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_load_contacts, dim3((nc + 255) / 256), dim3(256), 0, E->stream, t,
-                           nc, ci_d, cj_d, a_d, b_d, c_d, d_d, e_d, slot_d);
+                           nc, ci_d, cj_d, ej_d, a_d, b_d, c_d, d_d, e_d, slot_d);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipMemcpyAsync(t.row_start, start.data(), sizeof(int) * E->n, hipMemcpyHostToDevice, E->stream));
@@ -975,38 +1246,83 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
     return XDEM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Enqueue one step of every packing on the engine's stream.
+int launch_step(xdem_engine* E) {
+    const int gn = grid_of(E->n);
+    Table& old_t = E->T[E->cur];
+    Table& new_t = E->T[E->cur ^ 1];
+    const int par = (int)(E->steps & 1);
+    int* rb = E->rebuild_d + par;
+    int* rb_next = E->rebuild_d + (par ^ 1);
+    hipLaunchKernelGGL(k_bin, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d, E->B,
+                       old_t, new_t, rb, rb_next, E->err_d);
+    hipLaunchKernelGGL(k_vl_build, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
+                       E->B, old_t, new_t, rb, E->err_d, E->cnt_d);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (E->timing) {
+        while (E->ev_pool.size() < E->ev_used + 2) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            E->ev_pool.push_back(ev);
+        }
+        e0 = E->ev_pool[E->ev_used++];
+        e1 = E->ev_pool[E->ev_used++];
+        HIPCHK(hipEventRecord(e0, E->stream));
+    }
+    hipLaunchKernelGGL(k_pairs, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
+                       E->packs_d, old_t, new_t, rb, E->err_d, E->cnt_d);
+    if (E->timing) HIPCHK(hipEventRecord(e1, E->stream));
+    hipLaunchKernelGGL(k_gather, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t, E->err_d);
+    const int gi = std::max(gn, grid_of(E->B.nbins) / 4 + 1);
+    hipLaunchKernelGGL(k_integrate, dim3(gi), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
+                       E->B, old_t.counter, rb, rb_next, E->err_d);
+    HIPCHK(hipGetLastError());
+    E->cur ^= 1;
+    E->steps++;
+    return XDEM_OK;
+}
+
+// Capture two consecutive steps (starting at the current parity) as a graph.
+int capture_pair(xdem_engine* E) {
+    const int key = (int)(E->steps & 1);
+    const int cur0 = E->cur;
+    const long long steps0 = E->steps;
+    HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(E);
+    if (!rc) rc = launch_step(E);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(E->stream, &g);
+    E->cur = cur0;
+    E->steps = steps0;
+    if (rc) return rc;
+    if (ce != hipSuccess) return fail(E, XDEM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    HIPCHK(hipGraphInstantiate(&E->graph_exec[key], g, nullptr, nullptr, 0));
+    E->graph[key] = g;
+    return XDEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int xdem_step(xdem_engine* E, int nsteps) {
     if (!E || !E->have_particles) return fail(E, XDEM_E_INVALID, "set particles first");
     if (nsteps < 0) return fail(E, XDEM_E_INVALID, "nsteps < 0");
-    const int gn = grid_of(E->n);
-    for (int s = 0; s < nsteps; ++s) {
-        Table& old_t = E->T[E->cur];
-        Table& new_t = E->T[E->cur ^ 1];
-        hipLaunchKernelGGL(k_bin, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d, E->B, E->err_d);
-        hipLaunchKernelGGL(k_candidates, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
-                           E->B, old_t, new_t, E->err_d, E->cnt_d);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (E->timing) {
-            while (E->ev_pool.size() < E->ev_used + 2) {
-                hipEvent_t ev;
-                HIPCHK(hipEventCreate(&ev));
-                E->ev_pool.push_back(ev);
-            }
-            e0 = E->ev_pool[E->ev_used++];
-            e1 = E->ev_pool[E->ev_used++];
-            HIPCHK(hipEventRecord(e0, E->stream));
-        }
-        hipLaunchKernelGGL(k_pairs, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
-                           E->packs_d, old_t, new_t, E->err_d, E->cnt_d);
-        if (E->timing) HIPCHK(hipEventRecord(e1, E->stream));
-        hipLaunchKernelGGL(k_gather, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t, E->err_d);
-        const int gi = std::max(gn, grid_of(E->B.nbins) / 4 + 1);
-        hipLaunchKernelGGL(k_integrate, dim3(gi), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
-                           E->B, old_t.counter, E->err_d);
-        HIPCHK(hipGetLastError());
-        E->cur ^= 1;
-        E->steps++;
+    int left = nsteps;
+    int rc;
+    while (left >= 2 && E->use_graphs && !E->timing) {
+        const int key = (int)(E->steps & 1);
+        if (!E->graph_exec[key] && (rc = capture_pair(E))) return rc;
+        HIPCHK(hipGraphLaunch(E->graph_exec[key], E->stream));
+        E->steps += 2;                // two flips of E->cur cancel
+        left -= 2;
     }
+    for (; left > 0; --left)
+        if ((rc = launch_step(E))) return rc;
     return XDEM_OK;
 }
 
@@ -1029,24 +1345,19 @@ int xdem_get_particles(xdem_engine* E, double* x, double* y, double* theta, doub
                        double* vy, double* omega) {
     int rc = xdem_sync(E);
     if (rc) return rc;
-    const size_t db = sizeof(double) * (size_t)E->n;
-    if (x) HIPCHK(hipMemcpy(x, E->P.x, db, hipMemcpyDeviceToHost));
-    if (y) HIPCHK(hipMemcpy(y, E->P.y, db, hipMemcpyDeviceToHost));
-    if (theta) HIPCHK(hipMemcpy(theta, E->P.th, db, hipMemcpyDeviceToHost));
-    if (vx) HIPCHK(hipMemcpy(vx, E->P.vx, db, hipMemcpyDeviceToHost));
-    if (vy) HIPCHK(hipMemcpy(vy, E->P.vy, db, hipMemcpyDeviceToHost));
-    if (omega) HIPCHK(hipMemcpy(omega, E->P.w, db, hipMemcpyDeviceToHost));
+    if ((rc = fetch_ext(E, x, E->P.x)) || (rc = fetch_ext(E, y, E->P.y)) ||
+        (rc = fetch_ext(E, theta, E->P.th)) || (rc = fetch_ext(E, vx, E->P.vx)) ||
+        (rc = fetch_ext(E, vy, E->P.vy)) || (rc = fetch_ext(E, omega, E->P.w)))
+        return rc;
     return XDEM_OK;
 }
 
 int xdem_get_forces(xdem_engine* E, double* fx, double* fy, double* torque, int* num_contacts) {
     int rc = xdem_sync(E);
     if (rc) return rc;
-    const size_t db = sizeof(double) * (size_t)E->n;
-    if (fx) HIPCHK(hipMemcpy(fx, E->P.fx, db, hipMemcpyDeviceToHost));
-    if (fy) HIPCHK(hipMemcpy(fy, E->P.fy, db, hipMemcpyDeviceToHost));
-    if (torque) HIPCHK(hipMemcpy(torque, E->P.tq, db, hipMemcpyDeviceToHost));
-    if (num_contacts) HIPCHK(hipMemcpy(num_contacts, E->P.ncont, sizeof(int) * (size_t)E->n, hipMemcpyDeviceToHost));
+    if ((rc = fetch_ext(E, fx, E->P.fx)) || (rc = fetch_ext(E, fy, E->P.fy)) ||
+        (rc = fetch_ext(E, torque, E->P.tq)) || (rc = fetch_ext(E, num_contacts, E->P.ncont)))
+        return rc;
     return XDEM_OK;
 }
 
@@ -1058,22 +1369,21 @@ int xdem_get_contacts(xdem_engine* E, int cap, int* nc, int* ci, int* cj, double
     if (rc) return rc;
     if (!nc) return fail(E, XDEM_E_INVALID, "nc is NULL");
     const Table& t = E->T[E->cur];
-    int used = 0;
-    HIPCHK(hipMemcpy(&used, t.counter, sizeof(int), hipMemcpyDeviceToHost));
-    if (E->steps == 0) {          // restart table: rows only
-        std::vector<int> cnt(E->n);
-        HIPCHK(hipMemcpy(cnt.data(), t.row_cnt, sizeof(int) * E->n, hipMemcpyDeviceToHost));
-        used = 0;
-        for (int v : cnt) used += v;
-    }
-    used = std::min(used, t.cap);
+    std::vector<int> seg_used(kSeg, 0);
+    if ((rc = segment_counts(E, t, seg_used))) return rc;
+    const int used = t.cap;
     std::vector<int> pi(used), pj(used);
     HIPCHK(hipMemcpy(pi.data(), t.pr_i, sizeof(int) * used, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(pj.data(), t.ct_j, sizeof(int) * used, hipMemcpyDeviceToHost));
     std::vector<int> idx;
-    idx.reserve(used);
-    for (int e = 0; e < used; ++e)
-        if (pj[e] >= 0) idx.push_back(e);
+    idx.reserve(used / 4);
+    for (int e = 0; e < used; ++e) {
+        if (e % t.seg_cap >= seg_used[e / t.seg_cap]) continue;
+        if (pj[e] < 0) continue;
+        pi[e] = E->perm[pi[e]];
+        pj[e] = E->perm[pj[e]];
+        idx.push_back(e);
+    }
     std::sort(idx.begin(), idx.end(), [&](int a, int b) {
         return pi[a] != pi[b] ? pi[a] < pi[b] : pj[a] < pj[b];
     });
@@ -1112,8 +1422,8 @@ int xdem_get_stress(xdem_engine* E, double* out) {
     for (int k = 0; k < E->npacks; ++k) {
         double a = 0, b = 0, c = 0, d = 0;
         for (int p = E->pack_start[k]; p < E->pack_start[k + 1]; ++p) {
-            a += s[4 * (size_t)p]; b += s[4 * (size_t)p + 1];
-            c += s[4 * (size_t)p + 2]; d += s[4 * (size_t)p + 3];
+            const size_t q = 4 * (size_t)E->inv[p];
+            a += s[q]; b += s[q + 1]; c += s[q + 2]; d += s[q + 3];
         }
         const Pack& K = E->packs_h[k];
         const double area = K.xlen * K.ylen;
@@ -1136,17 +1446,23 @@ int xdem_get_boundary(xdem_engine* E, double* out) {
 
 int xdem_get_counters(xdem_engine* E, long long* newton_iters, long long* rescues,
                       long long* fallbacks, long long* candidates, long long* coarse_pairs,
-                      long long* steps) {
+                      long long* steps, long long* list_builds) {
     int rc = xdem_sync(E);
     if (rc) return rc;
-    Counters c;
-    HIPCHK(hipMemcpy(&c, E->cnt_d, sizeof c, hipMemcpyDeviceToHost));
+    std::vector<Counters> cs(kSeg);
+    HIPCHK(hipMemcpy(cs.data(), E->cnt_d, sizeof(Counters) * kSeg, hipMemcpyDeviceToHost));
+    Counters c{};
+    for (const Counters& q : cs) {
+        c.cand += q.cand; c.coarse += q.coarse; c.newton += q.newton;
+        c.rescues += q.rescues; c.fallbacks += q.fallbacks; c.builds += q.builds;
+    }
     if (newton_iters) *newton_iters = (long long)c.newton;
     if (rescues) *rescues = (long long)c.rescues;
     if (fallbacks) *fallbacks = (long long)c.fallbacks;
     if (candidates) *candidates = (long long)c.cand;
     if (coarse_pairs) *coarse_pairs = (long long)c.coarse;
     if (steps) *steps = E->steps;
+    if (list_builds) *list_builds = (long long)c.builds;
     return XDEM_OK;
 }
 
@@ -1172,13 +1488,13 @@ int xdem_get_traffic(xdem_engine* E, double* bytes_per_step, long long* contacts
     int rc = xdem_sync(E);
     if (rc) return rc;
     const Table& t = E->T[E->cur];
-    int used = 0;
-    HIPCHK(hipMemcpy(&used, t.counter, sizeof(int), hipMemcpyDeviceToHost));
-    used = std::min(used, t.cap);
-    std::vector<int> pj(used);
-    HIPCHK(hipMemcpy(pj.data(), t.ct_j, sizeof(int) * used, hipMemcpyDeviceToHost));
+    std::vector<int> seg_used(kSeg, 0);
+    if ((rc = segment_counts(E, t, seg_used))) return rc;
+    std::vector<int> pj(t.cap);
+    HIPCHK(hipMemcpy(pj.data(), t.ct_j, sizeof(int) * (size_t)t.cap, hipMemcpyDeviceToHost));
     long long live = 0;
-    for (int v : pj) live += v >= 0;
+    for (int e = 0; e < t.cap; ++e)
+        if (e % t.seg_cap < seg_used[e / t.seg_cap] && pj[e] >= 0) ++live;
     // state 96 B r/w + static 44 B per particle; 44 B history r/w per contact
     if (bytes_per_step) *bytes_per_step = 140.0 * E->n + 88.0 * (double)live;
     if (contacts) *contacts = live;

@@ -35,14 +35,14 @@ class XdemConfig(C.Structure):
         (k, C.c_double) for k in ("xmin", "xmax", "ymin", "ymax", "dt", "kn", "kt",
                                   "restitution", "mu", "global_damping", "shear_rate",
                                   "total_shift", "total_strain", "cell_size")] + [
-        ("newton_cap", C.c_int), ("strict_newton", C.c_int)]
+        ("newton_cap", C.c_int), ("strict_newton", C.c_int), ("skin", C.c_double)]
 
     @classmethod
     def from_world(cls, c: WorldConfig, precision: int = 0) -> "XdemConfig":
         return cls(c.world_kind, precision, c.xmin, c.xmax, c.ymin, c.ymax, c.dt, c.kn,
                    c.kt, c.restitution, c.mu, c.global_damping, c.shear_rate,
                    c.total_shift, c.total_strain, c.cell_size, c.newton_cap,
-                   c.strict_newton)
+                   c.strict_newton, getattr(c, "skin", 0.0))
 
 
 _D = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -74,7 +74,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_get_contacts.argtypes = [vp, C.c_int, C.POINTER(C.c_int)] + [vp] * 15
     L.xdem_get_stress.argtypes = [vp, _D]
     L.xdem_get_boundary.argtypes = [vp, _D]
-    L.xdem_get_counters.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 6
+    L.xdem_get_counters.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 7
     L.xdem_set_timing.argtypes = [vp, C.c_int]
     L.xdem_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.xdem_get_traffic.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
@@ -199,10 +199,10 @@ class Engine:
         return out.reshape(-1, 3)
 
     def counters(self):
-        v = [C.c_longlong(0) for _ in range(6)]
+        v = [C.c_longlong(0) for _ in range(7)]
         self._check(self.L.xdem_get_counters(self.h, *[C.byref(x) for x in v]))
         return dict(zip(("newton_iters", "rescues", "fallbacks", "candidates",
-                         "coarse_pairs", "steps"), [x.value for x in v]))
+                         "coarse_pairs", "steps", "list_builds"), [x.value for x in v]))
 
     def set_timing(self, on: bool):
         self._check(self.L.xdem_set_timing(self.h, 1 if on else 0))

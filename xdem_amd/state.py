@@ -57,6 +57,7 @@ class WorldConfig:
     precision: int = 0
     newton_cap: int = 0          # 0: 200 updates
     strict_newton: int = 0       # 1: error instead of the bisection rescue
+    skin: float = 0.0            # pair-list skin; 0: 0.2 * max outer radius
 
     def cfg_array(self) -> np.ndarray:
         return np.array([getattr(self, k) for k in _CFG_KEYS], dtype=np.float64)

@@ -113,9 +113,13 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = s.n * args.steps / elapsed
     kern_avg_ms = kern_ms / max(launches, 1)
-    achieved = bytes_step / (kern_avg_ms * 1e-3) / 1e9
+    # k_contact's algorithmic bytes: each particle's pose, velocity and shape
+    # read once (72 B) and its force and torque written once (24 B); each
+    # contact's history (xi, n, damping, partner) read and written (88 B)
+    kern_bytes = 96.0 * s.n + 88.0 * contacts
+    achieved = kern_bytes / (kern_avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_pairs_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "k_contact_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
     out = {
@@ -140,8 +144,9 @@ def main():
                    "pair_list_builds_timed": cnt["list_builds"] - builds0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pairs", "kernel_avg_ms": kern_avg_ms,
-                     "algorithmic_bytes_per_launch": bytes_step,
+                     "kernel": "k_contact", "kernel_avg_ms": kern_avg_ms,
+                     "algorithmic_bytes_per_launch": kern_bytes,
+                     "step_algorithmic_bytes": bytes_step,
                      "step_hbm_frac": bytes_step * args.steps / elapsed / (HBM_PEAK_GBS * 1e9)},
     }
     if not args.no_cpu:

@@ -95,53 +95,43 @@ __device__ __forceinline__ bool quartic_bisect(double A, double B, double C,
 // Quantities of one ellipse pair up to the quartic (ZPM, eqs 4-26 of Zheng
 // & Palffy-Muhoray, Phys. Rev. E 75, 061709), src/EllipticParticle.cpp:49-100.
 struct ZpmSetup {
-    double b1, e1s, b1s, dlen, dhx, dhy, k1x, k1y;
-    double Txx, Txy, Tyx, Tyy;
+    double b1, e1s, dlen, dhx, dhy, k1x, k1y;
     double kpx, kpy, kmx, kmy, a2p, b2p, delta;
     double A, B, C, D, E;
 };
 
+// P1's per-particle quantities of the ZPM test, evaluated once per step by
+// the operations of src/EllipticParticle.cpp:49-73 (T and I + k1k1 eta are
+// symmetric: their xy and yx entries are the same products).
+struct ZpmFrame {
+    double kx, ky;                  // k1 = (cos z1, sin z1)
+    double txx, txy, tyy;           // T = (I + k1k1 (b1/a1 - 1)) / b1
+    double mxx, mxy, myy;           // I + k1k1 (a1/b1 - 1)
+    double b, e2, bsq;              // b1, e1^2, b1^2
+};
+
 // Returns 0 when one of the transformed-frame culls (src/EllipticParticle.cpp:77-84)
-// rejects the pair, 1 when the quartic is ready.
-// fix_order = false orders the eigenvalues as the reference does (through
-// int abs); true orders them by |lambda|, as the algorithm intends.
-__device__ __forceinline__ int zpm_setup(double a1, double ar1, double z1,
-                                         double a2, double ar2, double z2,
-                                         double dx, double dy, ZpmSetup& S,
-                                         bool fix_order = false) {
-    const double b1 = a1 / ar1, b2 = a2 / ar2;
+// rejects the pair, 1 when the quartic is ready.  P1's frame is precomputed,
+// P2 enters through k2, e2^2 and b2^2; every operation and its order is the
+// reference's.  fix_order = false orders the eigenvalues as the reference
+// does (through int abs); true orders them by |lambda|, as the algorithm
+// intends.
+__device__ __forceinline__ int zpm_setup_frame(const ZpmFrame& f, double k2x, double k2y,
+                                               double e2s, double b2s, double dx, double dy,
+                                               ZpmSetup& S, bool fix_order = false) {
     const double dlen = sqrt(dx * dx + dy * dy);
     const double dhx = dx / dlen, dhy = dy / dlen;
-    double s1z, c1z, s2z, c2z;
-    sincos(z1, &s1z, &c1z);
-    sincos(z2, &s2z, &c2z);
-    const double k1x = c1z, k1y = s1z, k2x = c2z, k2y = s2z;
-    const double kxx = k1x * k1x, kxy = k1x * k1y, kyx = k1y * k1x, kyy = k1y * k1y;
-    const double qxx = k2x * k2x, qxy = k2x * k2y, qyx = k2y * k2x, qyy = k2y * k2y;
-    const double e1 = sqrt(1.0 - (b1 * b1) / (a1 * a1));
-    const double e2 = sqrt(1.0 - (b2 * b2) / (a2 * a2));
-    // T = (I + k1k1*(b1/a1-1)) / b1
-    const double s1 = b1 / a1 - 1.0;
-    const double Txx = (1.0 + kxx * s1) / b1, Txy = (0.0 + kxy * s1) / b1;
-    const double Tyx = (0.0 + kyx * s1) / b1, Tyy = (1.0 + kyy * s1) / b1;
-    const double eta = a1 / b1 - 1.0;
-    const double dphx = Txx * dhx + Txy * dhy, dphy = Tyx * dhx + Tyy * dhy;
-    const double dp0x = Txx * dx + Txy * dy, dp0y = Tyx * dx + Tyy * dy;
-    // A' = (I+k1k1*eta)(I-k2k2*e2^2)(I+k1k1*eta) * b1^2 / b2^2, with the
-    // product of src/MathMatrix2d.h:85-91 (its YY term uses o.yx)
-    const double m1xx = 1.0 + kxx * eta, m1xy = 0.0 + kxy * eta;
-    const double m1yx = 0.0 + kyx * eta, m1yy = 1.0 + kyy * eta;
-    const double e2s = e2 * e2;
-    const double mbxx = 1.0 - qxx * e2s, mbxy = 0.0 - qxy * e2s;
-    const double mbyx = 0.0 - qyx * e2s, mbyy = 1.0 - qyy * e2s;
-    const double pxx = m1xx * mbxx + m1xy * mbyx, pxy = m1xx * mbxy + m1xy * mbyy;
-    const double pyx = m1yx * mbxx + m1yy * mbyx, pyy = m1yx * mbyx + m1yy * mbyy;
-    double axx = pxx * m1xx + pxy * m1yx, axy = pxx * m1xy + pxy * m1yy;
-    double ayx = pyx * m1xx + pyy * m1yx, ayy = pyx * m1yx + pyy * m1yy;
-    const double b1s = b1 * b1, b2s = b2 * b2;
+    const double qxx = k2x * k2x, qxy = k2x * k2y, qyy = k2y * k2y;
+    const double dphx = f.txx * dhx + f.txy * dhy, dphy = f.txy * dhx + f.tyy * dhy;
+    const double dp0x = f.txx * dx + f.txy * dy, dp0y = f.txy * dx + f.tyy * dy;
+    const double mbxx = 1.0 - qxx * e2s, mbxy = 0.0 - qxy * e2s, mbyy = 1.0 - qyy * e2s;
+    const double pxx = f.mxx * mbxx + f.mxy * mbxy, pxy = f.mxx * mbxy + f.mxy * mbyy;
+    const double pyx = f.mxy * mbxx + f.myy * mbxy, pyy = f.mxy * mbxy + f.myy * mbyy;
+    double axx = pxx * f.mxx + pxy * f.mxy, axy = pxx * f.mxy + pxy * f.myy;
+    double ayx = pyx * f.mxx + pyy * f.mxy, ayy = pyx * f.mxy + pyy * f.myy;
+    const double b1s = f.bsq;
     axx = axx * b1s; axy = axy * b1s; ayx = ayx * b1s; ayy = ayy * b1s;
     axx = axx / b2s; axy = axy / b2s; ayx = ayx / b2s; ayy = ayy / b2s;
-    // matrix2d::eigen (src/MathMatrix2d.h:112-135)
     const double tr = axx + ayy;
     const double det = axx * ayy - axy * ayx;
     double disc = tr * tr - 4.0 * det;
@@ -154,14 +144,12 @@ __device__ __forceinline__ int zpm_setup(double a1, double ar1, double z1,
     else { lx = ev2; ly = ev1; }
     double kpx = 1.0, kpy = (lx - axx) / axy;
     {
-        const double ls = kpx * kpx + kpy * kpy;
-        const double s = sqrt(ls);     // ls >= 1 or NaN: never zero
+        const double s = sqrt(kpx * kpx + kpy * kpy);
         kpx /= s; kpy /= s;
     }
     double kmx = 1.0, kmy = (ly - axx) / axy;
     {
-        const double ls = kmx * kmx + kmy * kmy;
-        const double s = sqrt(ls);
+        const double s = sqrt(kmx * kmx + kmy * kmy);
         kmx /= s; kmy /= s;
     }
     const double a2p = 1.0 / sqrt(ly);
@@ -178,20 +166,22 @@ __device__ __forceinline__ int zpm_setup(double a1, double ar1, double z1,
     S.C = -t2 - (1.0 + delta) * (1.0 + delta) + (1.0 + (1.0 + delta) * t2) / (b2p * b2p);
     S.D = 2.0 * (1.0 + t2) * (1.0 + delta) / b2p;
     S.E = (1.0 + t2 + delta) * (1.0 + delta);
-    S.b1 = b1; S.e1s = e1 * e1; S.b1s = b1s; S.dlen = dlen; S.dhx = dhx; S.dhy = dhy;
-    S.k1x = k1x; S.k1y = k1y;
-    S.Txx = Txx; S.Txy = Txy; S.Tyx = Tyx; S.Tyy = Tyy;
+    S.b1 = f.b; S.e1s = f.e2; S.dlen = dlen; S.dhx = dhx; S.dhy = dhy;
+    S.k1x = f.kx; S.k1y = f.ky;
     S.kpx = kpx; S.kpy = kpy; S.kmx = kmx; S.kmy = kmy;
     S.a2p = a2p; S.b2p = b2p; S.delta = delta;
     return 1;
 }
 
-// From the quartic root q to the contact geometry, src/EllipticParticle.cpp:166-221.
-// (xi, yi) is P1's centre, (xj2, yj2) the image centre of P2.  Returns 0 no
-// contact, 1 contact, -1 zero normal.
-__device__ __forceinline__ int zpm_finish(const ZpmSetup& S, double q, double xi,
-                                          double yi, double xj2, double yj2,
-                                          double dx, double dy, Geom& g) {
+// From the quartic root q to the contact geometry, src/EllipticParticle.cpp:166-221,
+// with P1's T^-1 (ixx, ixy = iyx, iyy) and A1 = (I - e1^2 k1k1)/b1^2 (axx,
+// axy = ayx, ayy) precomputed.  (xi, yi) is P1's centre, (xj2, yj2) the image
+// centre of P2.  Returns 0 no contact, 1 contact, -1 zero normal.
+__device__ __forceinline__ int zpm_finish_frame(const ZpmSetup& S, double q, double xi,
+                                                double yi, double xj2, double yj2,
+                                                double dx, double dy, double ixx,
+                                                double ixy, double iyy, double axx,
+                                                double axy, double ayy, Geom& g) {
     const double delta = S.delta;
     double temp = (q * q - 1.0) / delta;
     double dprime;
@@ -219,18 +209,10 @@ __device__ __forceinline__ int zpm_finish(const ZpmSetup& S, double q, double xi
         wx = S.kmx * sx + S.kpx * sy;
         wy = S.kmy * sx + S.kpy * sy;
     }
-    const double tdet = S.Txx * S.Tyy - S.Txy * S.Tyx;
-    const double ixx = S.Tyy / tdet, ixy = -S.Txy / tdet;
-    const double iyx = -S.Tyx / tdet, iyy = S.Txx / tdet;
-    const double tpx = ixx * wx + ixy * wy, tpy = iyx * wx + iyy * wy;
+    const double tpx = ixx * wx + ixy * wy, tpy = ixy * wx + iyy * wy;
     const double dl2 = sqrt(dx * dx + dy * dy);
     const double cpx = tpx * dl2 / dist, cpy = tpy * dl2 / dist;
-    const double e1s = S.e1s, b1s = S.b1s;
-    const double kxx = S.k1x * S.k1x, kxy = S.k1x * S.k1y;
-    const double kyx = S.k1y * S.k1x, kyy = S.k1y * S.k1y;
-    const double Axx = (1.0 - kxx * e1s) / b1s, Axy = (0.0 - kxy * e1s) / b1s;
-    const double Ayx = (0.0 - kyx * e1s) / b1s, Ayy = (1.0 - kyy * e1s) / b1s;
-    double nx = Axx * tpx + Axy * tpy, ny = Ayx * tpx + Ayy * tpy;
+    double nx = axx * tpx + axy * tpy, ny = axy * tpx + ayy * tpy;
     {
         const double ls = nx * nx + ny * ny;
         if (ls != 0.0) { const double s = sqrt(ls); nx /= s; ny /= s; }

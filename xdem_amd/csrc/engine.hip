@@ -1,16 +1,20 @@
 // engine.hip -- MI355X (gfx950) DEM step engine behind the C-ABI of
 // include/xdem.h.
 //
-// One step of every packing is five launches on the engine's stream:
-//   k_bin         wrap positions (modifyParticlePosition), cell index; on a
-//                 list-rebuild step also fills fixed-capacity cell bins and
-//                 the two ghost rows of the Lees-Edwards boundary
+// One step of every packing is six launches on the engine's stream:
+//   k_bin         wrap positions (modifyParticlePosition), cell index, the
+//                 per-particle ellipse frame of this step; on a list-rebuild
+//                 step also fills fixed-capacity cell bins and the two ghost
+//                 rows of the Lees-Edwards boundary
 //   k_vl_build    (rebuild steps only) 3x3 stencil of the cell grid with the
 //                 reference's candidate filters; keeps pairs whose bounding
-//                 circles come within a skin; block-compacted pair list
-//   k_pairs       one lane per listed pair: the reference's candidate rule
+//                 circles come within a skin; rows sorted by partner ID,
+//                 block-compacted pair list, per-particle incoming index
+//   k_coarse      one lane per listed pair: the reference's candidate rule
 //                 and image offset from this step's cells, bounding-circle
-//                 test, ZPM / circle geometry, history, contact model
+//                 test; erases / carries history; compacts the survivors
+//   k_contact     one lane per surviving pair: ZPM / circle geometry,
+//                 history, contact model
 //                 (BaseParticle::calculateContactForceAndTorque)
 // The pair list is rebuilt when 2 x (largest displacement of a particle
 // since the last rebuild) + (change of the LE shift) reaches the skin, so
@@ -44,7 +48,6 @@ constexpr int kImgShift = 30;
 // workgroups allocating rows never queue on a single atomic address.
 constexpr int kSeg = 64;
 constexpr int kCtrStride = 32;
-constexpr int kImgStale = 3;
 
 struct Pack {
     double xmin, xmax, ymin, ymax, xlen, ylen, lenx, leny, hx, hy;
@@ -54,6 +57,19 @@ struct Pack {
     double vlenx, vleny;          // list grid: cells >= 2 r_max + skin
     int nx, ny, bin_base, world, p_begin, p_end, newton_cap, strict;
     int vnx, vny;                 // list grid size; bins hold (vny+2)*vnx cells
+};
+
+// Per-particle ellipse quantities.  Fixed: b = a/AR, e^2, b^2, b/a - 1 and
+// a/b - 1.  Per step (from theta after the wrap): the orientation k and,
+// for the particle's role as P1, the matrices T = (I + kk (b/a-1))/b, T^-1,
+// A = (I - e^2 kk)/b^2 and I + kk (a/b-1); each is symmetric, so three
+// entries.  These are the per-particle subexpressions of
+// src/EllipticParticle.cpp:49-73 and :209-216, evaluated once per step by the
+// same operations, so every pair reads the bits it would have computed.
+struct Frame {
+    const double *b, *e2, *bsq, *s1, *eta;
+    double *kx, *ky;
+    double *txx, *txy, *tyy, *ixx, *ixy, *iyy, *axx, *axy, *ayy, *mxx, *mxy, *myy;
 };
 
 struct Parts {
@@ -67,9 +83,12 @@ struct Parts {
     const int* pack;
     double *fx, *fy, *tq;
     int* ncont;
-    int *bin_of, *gbin_of;
-    int *rc_of, *grc_of;          // reference grid: (bin row << 16) | column
+    // reference grid cell as (bin row << 16) | column: x the particle's own,
+    // y its Lees-Edwards ghost copy (-1 when it has none)
+    int2* cell;
     int* vrc_of;                  // list grid cell of the last rebuild
+    // listed pairs held by lower IDs, per particle, as (ID of P1, slot),
+    // sorted by ID; rebuilt with the pair list
     int* in_cnt;
     int2* in_ref;
     double *dsx, *dsy;            // displacement since the list rebuild (a
@@ -77,15 +96,34 @@ struct Parts {
     int n;
 };
 
+// Force record of a contact, read by the per-particle sums: F = Fn + Fs on
+// P1 and the torques b1 x F, b2 x (-F) (src/ContactInformation.h:62-68).
+struct alignas(32) FRec {
+    double fx, fy, t1, t2;
+};
+
+// Pair / contact table.  Rows hold a particle's listed partners j > i in
+// ascending ID order; ct_j < 0 marks a listed pair without a contact.
 struct Table {
     int *pr_i, *pr_j, *ct_j;
     int* ej;                      // external index of the partner (ordering key)
     double *xix, *xiy, *nx, *ny, *c;
-    double *fnx, *fny, *fsx, *fsy, *t1, *t2, *bx, *by, *px, *py;
+    FRec* fr;
+    double *fnx, *fny, *fsx, *fsy, *bx, *by, *px, *py;
     int *row_start, *row_cnt;
     int* counter;                 // kSeg counters, kCtrStride apart
     int cap;                      // total entries
     int seg_cap;                  // entries per segment
+};
+
+// Pairs that passed the bounding-circle test this step: x = slot | (image
+// list << kLstShift), y = slot of the pair's history in the previous table
+// or -1.  Segmented like the table.
+constexpr int kLstShift = 30;
+struct WorkList {
+    int2* item;
+    int* counter;                 // kSeg counters, kCtrStride apart
+    int seg_cap;
 };
 
 struct Bins {
@@ -149,13 +187,32 @@ __device__ __forceinline__ void image_offset(const Pack& K, int img, double xi, 
 // cell bins and the two ghost rows are filled; otherwise the pair list
 // (rows and segment counters) carries over to the new table.
 __global__ void __launch_bounds__(kBlock)
-k_bin(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
-      const int* __restrict__ rebuild, int* rebuild_next, uint32_t* err) {
+k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
+      WorkList W, const int* __restrict__ rebuild, int* rebuild_next, uint32_t* err) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p == 0) *rebuild_next = 0;      // set again by k_integrate when needed
+    if (p < kSeg) W.counter[p * kCtrStride] = 0;
     if (p >= P.n || *err) return;
     const bool rb = *rebuild != 0;
     if (!rb && p < kSeg) new_t.counter[p * kCtrStride] = old_t.counter[p * kCtrStride];
+    if (P.kind[p] == KIND_ELLIPSE) {
+        // src/EllipticParticle.cpp:53-73 (P1: k1, T, I + k1k1 eta) and
+        // :211-216 (T^-1, A1); P2's I - k2k2 e2^2 is formed per pair from k
+        double s, c;
+        sincos(P.th[p], &s, &c);
+        const double kxx = c * c, kxy = c * s, kyy = s * s;
+        const double b = F.b[p], s1 = F.s1[p], eta = F.eta[p], e2 = F.e2[p], bsq = F.bsq[p];
+        const double txx = (1.0 + kxx * s1) / b, txy = (0.0 + kxy * s1) / b;
+        const double tyy = (1.0 + kyy * s1) / b;
+        const double tdet = txx * tyy - txy * txy;
+        F.kx[p] = c; F.ky[p] = s;
+        F.txx[p] = txx; F.txy[p] = txy; F.tyy[p] = tyy;
+        F.ixx[p] = tyy / tdet; F.ixy[p] = -txy / tdet; F.iyy[p] = txx / tdet;
+        F.axx[p] = (1.0 - kxx * e2) / bsq;
+        F.axy[p] = (0.0 - kxy * e2) / bsq;
+        F.ayy[p] = (1.0 - kyy * e2) / bsq;
+        F.mxx[p] = 1.0 + kxx * eta; F.mxy[p] = 0.0 + kxy * eta; F.myy[p] = 1.0 + kyy * eta;
+    }
     const Pack K = packs[P.pack[p]];
     double x = P.x[p], y = P.y[p];
     const double x0 = x, y0 = y;
@@ -169,12 +226,11 @@ k_bin(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     if (x < K.xmin) x += K.xlen;
     else if (x > K.xmax) x += -K.xlen;
     if (x != x0 || y != y0) { P.x[p] = x; P.y[p] = y; }
-    P.in_cnt[p] = 0;
+    if (rb) P.in_cnt[p] = 0;
     int ix = (int)floor((x - K.xmin) / K.lenx);
     int iy = (int)floor((y - K.ymin) / K.leny);
     ix = min(max(ix, 0), K.nx - 1);
     iy = min(max(iy, 0), K.ny - 1);
-    P.rc_of[p] = ((iy + 1) << 16) | ix;
     int grc = -1;
     int gx = ix;
     const int grow = iy == 0 ? K.ny + 1 : 0;
@@ -192,7 +248,7 @@ k_bin(Parts P, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
         }
         grc = (grow << 16) | gx;
     }
-    P.grc_of[p] = grc;
+    P.cell[p] = make_int2(((iy + 1) << 16) | ix, grc);
     if (!rb) {
         new_t.row_start[p] = old_t.row_start[p];
         new_t.row_cnt[p] = old_t.row_cnt[p];
@@ -236,33 +292,50 @@ __device__ __forceinline__ bool in_window(int nx, int rc_i, int rc_j) {
     return dc >= -1 && dc <= 1;
 }
 
-// The reference's candidate rule for P1 = i, P2 = j this step
-// (src/PeriodicBoundaryWorld.cpp:47-84, src/LeesEdwardsBoundaryWorld.cpp:128-204):
-// -1 when j is not in i's candidate lists, otherwise the list it is in:
+// The cell part of the reference's candidate rule for P1 = i, P2 = j this
+// step (src/PeriodicBoundaryWorld.cpp:58-81, src/LeesEdwardsBoundaryWorld.cpp:141-199):
+// -1 when j is in none of i's candidate lists, otherwise the list it is in:
 // 0 the normal list (minimum image), 1 the LE list of the ghost row above
 // the grid, 2 the LE list of the ghost row below.
-__device__ __forceinline__ int candidate_list(const Parts& P, const Pack& K, int i, int j) {
-    if (K.world == WORLD_PERIODIC &&
-        ((P.flags[i] | P.flags[j]) & SKIP_CONTACT_DETECTION))
-        return -1;
-    if (P.eid[i] >= P.eid[j] || P.up[i] == P.up[j]) return -1;
-    const int ri = P.rc_of[i];
-    if (in_window(K.nx, ri, P.rc_of[j])) return 0;
-    const int g = P.grc_of[j];
-    if (g >= 0 && in_window(K.nx, ri, g)) {
+__device__ __forceinline__ int cell_list(const Parts& P, const Pack& K, int i, int j) {
+    const int ri = P.cell[i].x;
+    const int2 cj = P.cell[j];
+    if (in_window(K.nx, ri, cj.x)) return 0;
+    if (cj.y >= 0 && in_window(K.nx, ri, cj.y)) {
         if (K.world == WORLD_PERIODIC) return 0;
-        return (g >> 16) == 0 ? 2 : 1;
+        return (cj.y >> 16) == 0 ? 2 : 1;
     }
     return -1;
 }
+
+// The ID, clump and flag filters of the candidate rule
+// (src/PeriodicBoundaryWorld.cpp:71-72; SKIP_CONTACT_DETECTION in the
+// periodic world only).  They do not change during a run.
+__device__ __forceinline__ bool static_candidate(const Parts& P, const Pack& K, int i, int j) {
+    if (K.world == WORLD_PERIODIC && ((P.flags[i] | P.flags[j]) & SKIP_CONTACT_DETECTION))
+        return false;
+    return P.eid[i] < P.eid[j] && P.up[i] != P.up[j];
+}
+
+__device__ __forceinline__ int candidate_list(const Parts& P, const Pack& K, int i, int j) {
+    return static_candidate(P, K, i, j) ? cell_list(P, K, i, j) : -1;
+}
+
+// A row entry added at a rebuild from the history rather than from the
+// stencil carries this bit on its partner index: for it the static filters
+// and the shapes are checked every step.
+constexpr int kExtraBit = 1 << 30;
 
 // Pair-list rebuild: one thread per particle i walks the 3x3 stencil of the
 // cell bins (src/PeriodicBoundaryWorld.cpp:58-81, Lees-Edwards ghost rows
 // src/LeesEdwardsBoundaryWorld.cpp:141-199) with the reference's filters and
 // keeps pairs whose bounding circles are closer than the skin; the
 // reference's per-candidate checks (>= 200 candidates, unlike shapes) run
-// here.  History pairs of i missing from the new list are appended so that
-// the pair kernel can decide between erasing and carrying them forward.
+// here.  History pairs of i missing from the new list are added so that the
+// pair kernels can decide between erasing and carrying them forward.  A row
+// is sorted by the partners' IDs, the order in which the reference's
+// ordered contact maps are summed, and each pair is entered in its
+// partner's incoming index.
 constexpr int kMaxList = 48;       // listed pairs j>i per particle
 
 __global__ void __launch_bounds__(kBlock)
@@ -276,7 +349,7 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     if (blockIdx.x == 0 && tid == 0) atomicAdd(&cnt->builds, 1ULL);
     const int i = blockIdx.x * kBlock + tid;
     const bool active = i < P.n && *err == 0;
-    int nl = 0, nextra = 0;
+    int nl = 0;
     unsigned long long ncand_total = 0;
     if (active) {
         const Pack K = packs[P.pack[i]];
@@ -328,19 +401,31 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
                 raise_err(err, ERR_CAND_OVERFLOW);
             ncand_total = (unsigned long long)(ncand[0] + ncand[1] + ncand[2]);
         }
-        if (nl > kMaxList) { raise_err(err, ERR_ROW_OVERFLOW); nl = kMaxList; }
+        const int nlisted = min(nl, kMaxList);
         const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
         for (int e = os; e < os + oc; ++e) {
             const int j = old_t.ct_j[e];
             if (j < 0) continue;
             bool listed = false;
-            for (int k = 0; k < nl; ++k) listed |= s_list[k][tid] == j;
-            if (!listed) ++nextra;
+            for (int k = 0; k < nlisted; ++k) listed |= s_list[k][tid] == j;
+            if (listed) continue;
+            if (nl < kMaxList) s_list[nl][tid] = j | kExtraBit;
+            ++nl;
+        }
+        if (nl > kMaxList) { raise_err(err, ERR_ROW_OVERFLOW); nl = kMaxList; }
+        for (int a = 1; a < nl; ++a) {          // ascending partner ID
+            const int v = s_list[a][tid];
+            const int kv = P.eid[v & ~kExtraBit];
+            int b = a - 1;
+            while (b >= 0 && P.eid[s_list[b][tid] & ~kExtraBit] > kv) {
+                s_list[b + 1][tid] = s_list[b][tid];
+                --b;
+            }
+            s_list[b + 1][tid] = v;
         }
     }
     count_add(&counter_slot(cnt)->cand, ncand_total);
-    const int total = nl + nextra;
-    s_scan[tid] = total;
+    s_scan[tid] = nl;
     __syncthreads();
     for (int o = 1; o < kBlock; o <<= 1) {
         const int v = tid >= o ? s_scan[tid - o] : 0;
@@ -364,270 +449,384 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     if (!active) return;
     const int base = s_base;
     if (base < 0) { new_t.row_start[i] = 0; new_t.row_cnt[i] = 0; return; }
-    const int s0 = base + s_scan[tid] - total;
+    const int s0 = base + s_scan[tid] - nl;
     new_t.row_start[i] = s0;
-    new_t.row_cnt[i] = total;
+    new_t.row_cnt[i] = nl;
+    const int eidi = P.eid[i];
     for (int k = 0; k < nl; ++k) {
+        const int code = s_list[k][tid];
+        const int j = code & ~kExtraBit;
         new_t.pr_i[s0 + k] = i;
-        new_t.pr_j[s0 + k] = s_list[k][tid];
-    }
-    if (nextra) {
-        int k = nl;
-        const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
-        for (int e = os; e < os + oc; ++e) {
-            const int j = old_t.ct_j[e];
-            if (j < 0) continue;
-            bool listed = false;
-            for (int q = 0; q < nl; ++q) listed |= s_list[q][tid] == j;
-            if (listed) continue;
-            new_t.pr_i[s0 + k] = i;
-            new_t.pr_j[s0 + k] = j;
-            ++k;
-        }
+        new_t.pr_j[s0 + k] = code;
+        const int q = atomicAdd(&P.in_cnt[j], 1);
+        if (q < kInCap) P.in_ref[(size_t)q * P.n + j] = make_int2(eidi, s0 + k);
+        else raise_err(err, ERR_INCOMING_OVERFLOW);
     }
 }
 
+// Prefix of the kSeg segment counts (capped at the segment size) into
+// s_pre[0..kSeg]; thread 0 of the block.
+__device__ __forceinline__ void segment_prefix(int* s_pre, const int* ctr, int seg_cap) {
+    int acc = 0;
+    for (int g = 0; g < kSeg; ++g) {
+        s_pre[g] = acc;
+        acc += min(ctr[g * kCtrStride], seg_cap);
+    }
+    s_pre[kSeg] = acc;
+}
+
+// Flat index w -> segment lo with s_pre[lo] <= w < s_pre[lo + 1].
+__device__ __forceinline__ int segment_of(const int* s_pre, int w) {
+    int lo = 0, hi = kSeg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= w) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
-// One lane per listed pair (i, j): the candidate rule and image of this step,
-// bounding-circle test (src/BaseParticle.h:131-132), contact geometry,
-// history and contact model (src/BaseParticle.cpp:27-98).  A history entry
-// whose pair is no longer a candidate is carried forward untouched
-// (src/BaseWorld.cpp:34-46); a candidate out of contact is erased.
-__global__ void __launch_bounds__(kBlock, 2)
-k_pairs(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t,
-        const int* __restrict__ rebuild, uint32_t* err, Counters* cnt) {
+// One lane per listed pair (i, j): the reference's candidate rule and image
+// of this step, then the bounding-circle test (src/BaseParticle.h:131-132),
+// or for circles the overlap test of src/CircularParticle.cpp:73-76.  A
+// history entry whose pair is no longer a candidate is carried forward
+// untouched (src/BaseWorld.cpp:34-46); a candidate that fails the test is
+// erased (src/BaseParticle.cpp:93-97); the others go to the contact kernel
+// as block-compacted work items.
+__global__ void __launch_bounds__(kBlock)
+k_coarse(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, WorkList W,
+         const int* __restrict__ rebuild, uint32_t* err, Counters* cnt) {
     __shared__ int s_pre[kSeg + 1];
-    if (*err) return;
+    __shared__ int s_wave[kBlock / 64];
+    __shared__ int s_base;
+    __shared__ int s_go;
     const bool rb = *rebuild != 0;
     const int* __restrict__ src_i = rb ? new_t.pr_i : old_t.pr_i;
     const int* __restrict__ src_j = rb ? new_t.pr_j : old_t.pr_j;
-    const int* src_ctr = rb ? new_t.counter : old_t.counter;
     if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int g = 0; g < kSeg; ++g) {
-            s_pre[g] = acc;
-            acc += min(src_ctr[g * kCtrStride], new_t.seg_cap);
-        }
-        s_pre[kSeg] = acc;
+        s_go = *err == 0;
+        segment_prefix(s_pre, rb ? new_t.counter : old_t.counter, new_t.seg_cap);
     }
     __syncthreads();
+    if (!s_go) return;
     const int total = s_pre[kSeg];
-    unsigned long long n_newton = 0, n_rescue = 0, n_coarse = 0;
-    for (int w = blockIdx.x * kBlock + threadIdx.x; w < total; w += gridDim.x * kBlock) {
-        int lo = 0, hi = kSeg;                 // s_pre[lo] <= w < s_pre[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_pre[mid] <= w) lo = mid; else hi = mid;
-        }
-        const int e = lo * new_t.seg_cap + (w - s_pre[lo]);
-        const int i = src_i[e];
-        const int j = src_j[e];
-        if (!rb) { new_t.pr_i[e] = i; new_t.pr_j[e] = j; }
-        if (i < 0 || i >= P.n || j < 0 || j >= P.n) { new_t.ct_j[e] = -1; continue; }
-        const Pack K = packs[P.pack[i]];
-        // history of (i, j): the same slot when the list was not rebuilt
-        int found = -1;
-        if (!rb) {
-            if (old_t.ct_j[e] == j) found = e;
-        } else {
-            const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
-            for (int t = os; t < os + oc; ++t)
-                if (old_t.ct_j[t] == j) { found = t; break; }
-        }
-        const int lst = candidate_list(P, K, i, j);
-        const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
-        double ox = 0.0, oy = 0.0, ovx = 0.0;
-        Geom g;
-        int hit = 0;
-        if (lst >= 0 && P.kind[i] != KIND_CLUMP && P.kind[j] == P.kind[i]) {
-            image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
-            const double xj2 = xj + ox, yj2 = yj + oy;
-            const double dx = xj2 - xi, dy = yj2 - yi;
-            if (P.kind[i] == KIND_CIRCLE) {
-                ++n_coarse;
-                hit = circle_contact(xi, yi, P.r[i], xj2, yj2, P.r[j], g) ? 1 : 0;
-            } else {
-                const double qx = xi - (xj + ox), qy = yi - (yj + oy);
-                if ((P.r[i] + P.r[j]) > sqrt(qx * qx + qy * qy)) {
-                    ++n_coarse;
-                    // attempt 0: the reference's arithmetic; attempt 1 only
-                    // when that gives no valid answer (non-finite contact
-                    // geometry): eigenvalues ordered by |lambda|
-                    for (int attempt = 0; attempt < 2; ++attempt) {
-                        ZpmSetup S;
-                        hit = zpm_setup(P.r[i], P.ar[i], P.th[i], P.r[j], P.ar[j], P.th[j], dx,
-                                        dy, S, attempt == 1);
-                        if (!hit) break;
-                        const double qinit = sqrt(S.delta + 1.0);
-                        double q;
-                        int it = newton_quartic(S.A, S.B, S.C, S.D, S.E, qinit, K.newton_cap, &q);
-                        if (it < 0) {
-                            if (K.strict || !quartic_bisect(S.A, S.B, S.C, S.D, S.E, qinit, &q)) {
-                                hit = -2;
-                                break;
-                            }
-                            ++n_rescue;
-                            it = K.newton_cap;
-                        }
-                        n_newton += (unsigned long long)it;
-                        hit = zpm_finish(S, q, xi, yi, xj2, yj2, dx, dy, g);
-                        if (hit == 1 && attempt == 0 &&
-                            !(isfinite(g.overlap) && isfinite(g.nx) && isfinite(g.ny) &&
-                              isfinite(g.px) && isfinite(g.py))) {
-                            if (K.strict) { hit = -2; break; }
-                            ++n_rescue;
-                            continue;
-                        }
-                        break;
-                    }
-                    if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); new_t.ct_j[e] = -1; continue; }
-                    if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); new_t.ct_j[e] = -1; continue; }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long n_coarse = 0;
+    for (int w0 = blockIdx.x * kBlock; w0 < total; w0 += gridDim.x * kBlock) {
+        const int w = w0 + threadIdx.x;
+        bool push = false;
+        int2 item = make_int2(0, -1);
+        if (w < total) {
+            const int lo = segment_of(s_pre, w);
+            const int e = lo * new_t.seg_cap + (w - s_pre[lo]);
+            const int i = src_i[e];
+            const int code = src_j[e];
+            const int j = code & ~kExtraBit;
+            if (!rb) { new_t.pr_i[e] = i; new_t.pr_j[e] = code; }
+            if (i >= 0 && i < P.n && j >= 0 && j < P.n) {
+                const Pack K = packs[P.pack[i]];
+                // history of (i, j): the same slot when the list was not rebuilt
+                int found = -1;
+                if (!rb) {
+                    if (old_t.ct_j[e] == j) found = e;
+                } else {
+                    const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
+                    for (int t = os; t < os + oc; ++t)
+                        if (old_t.ct_j[t] == j) { found = t; break; }
                 }
-            }
-        }
-        if (!hit) {
-            if (found >= 0 && lst < 0) {           // not a candidate: carried forward
-                new_t.ct_j[e] = j;
-                new_t.ej[e] = old_t.ej[found];
-                new_t.xix[e] = old_t.xix[found]; new_t.xiy[e] = old_t.xiy[found];
-                new_t.nx[e] = old_t.nx[found];   new_t.ny[e] = old_t.ny[found];
-                new_t.c[e] = old_t.c[found];
-                new_t.fnx[e] = old_t.fnx[found]; new_t.fny[e] = old_t.fny[found];
-                new_t.fsx[e] = old_t.fsx[found]; new_t.fsy[e] = old_t.fsy[found];
-                new_t.t1[e] = old_t.t1[found];   new_t.t2[e] = old_t.t2[found];
-                new_t.bx[e] = old_t.bx[found];   new_t.by[e] = old_t.by[found];
-                new_t.px[e] = old_t.px[found];   new_t.py[e] = old_t.py[found];
-                const int q = atomicAdd(&P.in_cnt[j], 1);
-                if (q < kInCap) P.in_ref[(size_t)j * kInCap + q] = make_int2(P.eid[i], e);
-                else raise_err(err, ERR_INCOMING_OVERFLOW);
+                const int ki = P.kind[i];
+                int lst;
+                bool shapes = true;
+                if (code & kExtraBit) {
+                    lst = candidate_list(P, K, i, j);
+                    shapes = ki != KIND_CLUMP && P.kind[j] == ki;
+                } else {
+                    lst = cell_list(P, K, i, j);
+                }
+                if (lst >= 0 && shapes) {
+                    const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
+                    double ox, oy, ovx;
+                    image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
+                    if (ki == KIND_CIRCLE) {
+                        const double bvx = (xj + ox) - xi, bvy = (yj + oy) - yi;
+                        push = !((P.r[i] + P.r[j]) - sqrt(bvx * bvx + bvy * bvy) < 0.0);
+                        ++n_coarse;
+                    } else {
+                        const double qx = xi - (xj + ox), qy = yi - (yj + oy);
+                        push = (P.r[i] + P.r[j]) > sqrt(qx * qx + qy * qy);
+                        n_coarse += push ? 1 : 0;
+                    }
+                }
+                if (push) {
+                    item = make_int2((int)((unsigned)e | ((unsigned)lst << kLstShift)), found);
+                } else if (found >= 0 && lst < 0) {     // not a candidate: carried forward
+                    new_t.ct_j[e] = j;
+                    new_t.ej[e] = old_t.ej[found];
+                    new_t.xix[e] = old_t.xix[found]; new_t.xiy[e] = old_t.xiy[found];
+                    new_t.nx[e] = old_t.nx[found];   new_t.ny[e] = old_t.ny[found];
+                    new_t.c[e] = old_t.c[found];
+                    new_t.fr[e] = old_t.fr[found];
+                    new_t.fnx[e] = old_t.fnx[found]; new_t.fny[e] = old_t.fny[found];
+                    new_t.fsx[e] = old_t.fsx[found]; new_t.fsy[e] = old_t.fsy[found];
+                    new_t.bx[e] = old_t.bx[found];   new_t.by[e] = old_t.by[found];
+                    new_t.px[e] = old_t.px[found];   new_t.py[e] = old_t.py[found];
+                } else {
+                    new_t.ct_j[e] = -1;
+                }
             } else {
                 new_t.ct_j[e] = -1;
             }
-            continue;
         }
-        // relative velocity, src/BaseParticle.cpp:11-25; LE velocities carry
-        // the background term vx + rate*y (src/LeesEdwardsBoundaryWorld.cpp:508-518)
-        const bool le = K.world == WORLD_LE;
-        const double vxi = le ? P.vx[i] + K.rate * yi : P.vx[i];
-        const double vxj = le ? P.vx[j] + K.rate * yj : P.vx[j];
-        const double wi = P.w[i], wj = P.w[j];
-        const double ax = vxi + (-wi * g.b1y), ay = P.vy[i] + wi * g.b1x;
-        const double bx = (vxj + ovx) + (-wj * g.b2y), by = (P.vy[j] + 0.0) + wj * g.b2x;
-        const double rvx = ax - bx, rvy = ay - by;
-        const double sv = rvx * g.nx + rvy * g.ny;
-        const double nrvx = g.nx * sv, nrvy = g.ny * sv;
-        const double srvx = rvx - nrvx, srvy = rvy - nrvy;
-        double xix, xiy, c;
-        if (found < 0) {
-            // new contact, src/BaseParticle.cpp:38-64
-            const double vn = rvx * g.nx + rvy * g.ny;
-            if (!isnormal(vn)) { xix = srvx * K.dt; xiy = srvy * K.dt; }
-            else {
-                const double t = g.overlap / vn;
-                const double s = t < K.dt ? t : K.dt;
-                xix = srvx * s; xiy = srvy * s;
+        // block compaction: one counter update per block and round
+        const unsigned long long m = __ballot(push ? 1 : 0);
+        const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+        if (lane == 0) s_wave[wave] = __popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int k = 0; k < kBlock / 64; ++k) {
+                const int c = s_wave[k];
+                s_wave[k] = tot;
+                tot += c;
             }
-            // damping constant, src/ContactModel_LinearDissipativeFrictional.cpp:73-82
-            const bool si = P.flags[i] & SKIP_TIME_INTEGRAL, sj = P.flags[j] & SKIP_TIME_INTEGRAL;
-            if (!si && sj) c = sqrt(P.m[i]) * K.factor;
-            else if (si && !sj) c = sqrt(P.m[j]) * K.factor;
-            else {
-                const double m0 = (P.m[i] * P.m[j]) / (P.m[i] + P.m[j]);
-                c = sqrt(m0) * K.factor;
+            int base = -1;
+            if (tot > 0) {
+                const int seg = blockIdx.x % kSeg;
+                base = atomicAdd(W.counter + seg * kCtrStride, tot);
+                if ((long long)base + tot > W.seg_cap) {
+                    raise_err(err, ERR_PAIR_OVERFLOW);
+                    base = -1;
+                } else {
+                    base += seg * W.seg_cap;
+                }
             }
-        } else {
-            // persisting contact, src/BaseParticle.cpp:67-82
-            const double onx = old_t.nx[found], ony = old_t.ny[found];
-            const double ox0 = old_t.xix[found], oy0 = old_t.xiy[found];
-            const double dot = onx * g.nx + ony * g.ny;
-            const double det = onx * g.ny - ony * g.nx;
-            const double ang = atan2(det, dot);
-            double sa, ca;
-            sincos(ang, &sa, &ca);
-            const double rx = ca * ox0 + (-sa) * oy0;
-            const double ry = sa * ox0 + ca * oy0;
-            xix = rx + srvx * K.dt;
-            xiy = ry + srvy * K.dt;
-            c = old_t.c[found];
+            s_base = base;
         }
-        // contact model, src/ContactModel_LinearDissipativeFrictional.cpp:3-46
-        const double fnx = (-g.nx) * K.kn * g.overlap + (-nrvx) * c;
-        const double fny = (-g.ny) * K.kn * g.overlap + (-nrvy) * c;
-        const double fmax = sqrt(fnx * fnx + fny * fny) * K.mu;
-        const double ex = (-xix) * K.kt, ey = (-xiy) * K.kt;
-        double fsx = ex + (-srvx) * c, fsy = ey + (-srvy) * c;
-        if (sqrt(fsx * fsx + fsy * fsy) > fmax) {
-            const double ls = fsx * fsx + fsy * fsy;
-            double ux = 0.0, uy = 0.0;
-            if (ls != 0.0) { const double s = sqrt(ls); ux = fsx / s; uy = fsy / s; }
-            fsx = ux * fmax; fsy = uy * fmax;
-            if (sqrt(ex * ex + ey * ey) > fmax) { xix = (-fsx) / K.kt; xiy = (-fsy) / K.kt; }
-            else { xix = (-ex) / K.kt; xiy = (-ey) / K.kt; }
+        __syncthreads();
+        if (push && s_base >= 0) W.item[s_base + s_wave[wave] + before] = item;
+        __syncthreads();
+    }
+    count_add(&counter_slot(cnt)->coarse, n_coarse);
+}
+
+// ---------------------------------------------------------------------------
+// One pair that passed the bounding-circle test: contact geometry
+// (EllipticParticle::contactWithEllipticParticle, src/EllipticParticle.cpp:34-285;
+// CircularParticle::contactWithCircularParticle, src/CircularParticle.cpp:67-92),
+// history and contact model (BaseParticle::calculateContactForceAndTorque,
+// src/BaseParticle.cpp:27-98).
+__device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, const Pack& K,
+                                             const Table& old_t, const Table& new_t, int e,
+                                             int lst, int found, int i, int j, uint32_t* err,
+                                             unsigned long long& n_newton,
+                                             unsigned long long& n_rescue) {
+    const double xi = P.x[i], yi = P.y[i];
+    double ox, oy, ovx;
+    image_offset(K, lst, xi, yi, P.x[j], P.y[j], ox, oy, ovx);
+    const double xj2 = P.x[j] + ox, yj2 = P.y[j] + oy;
+    Geom g;
+    int hit;
+    if (P.kind[i] == KIND_CIRCLE) {
+        hit = circle_contact(xi, yi, P.r[i], xj2, yj2, P.r[j], g) ? 1 : 0;
+    } else {
+        const double dx = xj2 - xi, dy = yj2 - yi;
+        ZpmFrame f;
+        f.kx = F.kx[i]; f.ky = F.ky[i];
+        f.txx = F.txx[i]; f.txy = F.txy[i]; f.tyy = F.tyy[i];
+        f.mxx = F.mxx[i]; f.mxy = F.mxy[i]; f.myy = F.myy[i];
+        f.b = F.b[i]; f.e2 = F.e2[i]; f.bsq = F.bsq[i];
+        const double k2x = F.kx[j], k2y = F.ky[j], e2s = F.e2[j], b2s = F.bsq[j];
+        // attempt 0: the reference's arithmetic; attempt 1 only when that
+        // gives no valid answer (non-finite contact geometry): eigenvalues
+        // ordered by |lambda|
+        hit = 0;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            ZpmSetup S;
+            hit = zpm_setup_frame(f, k2x, k2y, e2s, b2s, dx, dy, S, attempt == 1);
+            if (!hit) break;
+            const double qinit = sqrt(S.delta + 1.0);
+            double q;
+            int its = newton_quartic(S.A, S.B, S.C, S.D, S.E, qinit, K.newton_cap, &q);
+            if (its < 0) {
+                if (K.strict || !quartic_bisect(S.A, S.B, S.C, S.D, S.E, qinit, &q)) {
+                    hit = -2;
+                    break;
+                }
+                ++n_rescue;
+                its = K.newton_cap;
+            }
+            n_newton += (unsigned long long)its;
+            hit = zpm_finish_frame(S, q, xi, yi, xj2, yj2, dx, dy, F.ixx[i], F.ixy[i],
+                                   F.iyy[i], F.axx[i], F.axy[i], F.ayy[i], g);
+            if (hit == 1 && attempt == 0 &&
+                !(isfinite(g.overlap) && isfinite(g.nx) && isfinite(g.ny) &&
+                  isfinite(g.px) && isfinite(g.py))) {
+                if (K.strict) { hit = -2; break; }
+                ++n_rescue;
+                continue;
+            }
+            break;
         }
-        // src/ContactInformation.h:62-68
-        const double fx = fnx + fsx, fy = fny + fsy;
-        new_t.ct_j[e] = j;
-        new_t.ej[e] = P.eid[j];
-        new_t.xix[e] = xix; new_t.xiy[e] = xiy;
-        new_t.nx[e] = g.nx; new_t.ny[e] = g.ny;
-        new_t.c[e] = c;
-        new_t.fnx[e] = fnx; new_t.fny[e] = fny;
-        new_t.fsx[e] = fsx; new_t.fsy[e] = fsy;
-        new_t.t1[e] = g.b1x * fy - g.b1y * fx;
-        new_t.t2[e] = g.b2x * (-fy) - g.b2y * (-fx);
-        new_t.bx[e] = xi - (xj + ox);
-        new_t.by[e] = yi - (yj + oy);
-        new_t.px[e] = g.px; new_t.py[e] = g.py;
-        const int q = atomicAdd(&P.in_cnt[j], 1);
-        if (q < kInCap) P.in_ref[(size_t)j * kInCap + q] = make_int2(P.eid[i], e);
-        else raise_err(err, ERR_INCOMING_OVERFLOW);
+        if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); new_t.ct_j[e] = -1; return; }
+        if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); new_t.ct_j[e] = -1; return; }
+    }
+    if (!hit) { new_t.ct_j[e] = -1; return; }
+    // relative velocity, src/BaseParticle.cpp:11-25; LE velocities carry
+    // the background term vx + rate*y (src/LeesEdwardsBoundaryWorld.cpp:508-518)
+    const bool le = K.world == WORLD_LE;
+    const double vxi = le ? P.vx[i] + K.rate * yi : P.vx[i];
+    const double vxj = le ? P.vx[j] + K.rate * P.y[j] : P.vx[j];
+    const double wi = P.w[i], wj = P.w[j];
+    const double ax = vxi + (-wi * g.b1y), ay = P.vy[i] + wi * g.b1x;
+    const double bx = (vxj + ovx) + (-wj * g.b2y), by = (P.vy[j] + 0.0) + wj * g.b2x;
+    const double rvx = ax - bx, rvy = ay - by;
+    const double sv = rvx * g.nx + rvy * g.ny;
+    const double nrvx = g.nx * sv, nrvy = g.ny * sv;
+    const double srvx = rvx - nrvx, srvy = rvy - nrvy;
+    double xix, xiy, c;
+    if (found < 0) {
+        // new contact, src/BaseParticle.cpp:38-64
+        const double vn = rvx * g.nx + rvy * g.ny;
+        if (!isnormal(vn)) { xix = srvx * K.dt; xiy = srvy * K.dt; }
+        else {
+            const double t = g.overlap / vn;
+            const double s = t < K.dt ? t : K.dt;
+            xix = srvx * s; xiy = srvy * s;
+        }
+        // damping constant, src/ContactModel_LinearDissipativeFrictional.cpp:73-82
+        const bool si = P.flags[i] & SKIP_TIME_INTEGRAL, sj = P.flags[j] & SKIP_TIME_INTEGRAL;
+        if (!si && sj) c = sqrt(P.m[i]) * K.factor;
+        else if (si && !sj) c = sqrt(P.m[j]) * K.factor;
+        else {
+            const double m0 = (P.m[i] * P.m[j]) / (P.m[i] + P.m[j]);
+            c = sqrt(m0) * K.factor;
+        }
+    } else {
+        // persisting contact, src/BaseParticle.cpp:67-82: xi turns by
+        // the angle atan2(det, dot) between the old and new normal; its
+        // cosine and sine are dot/h and det/h (h = |(dot, det)|), the same
+        // rotation to an ulp without the two transcendental calls
+        const double onx = old_t.nx[found], ony = old_t.ny[found];
+        const double ox0 = old_t.xix[found], oy0 = old_t.xiy[found];
+        const double dot = onx * g.nx + ony * g.ny;
+        const double det = onx * g.ny - ony * g.nx;
+        const double h = sqrt(dot * dot + det * det);
+        double sa = 0.0, ca = 1.0;            // atan2(0, 0) = 0
+        if (h != 0.0) { ca = dot / h; sa = det / h; }
+        const double rx = ca * ox0 + (-sa) * oy0;
+        const double ry = sa * ox0 + ca * oy0;
+        xix = rx + srvx * K.dt;
+        xiy = ry + srvy * K.dt;
+        c = old_t.c[found];
+    }
+    // contact model, src/ContactModel_LinearDissipativeFrictional.cpp:3-46
+    const double fnx = (-g.nx) * K.kn * g.overlap + (-nrvx) * c;
+    const double fny = (-g.ny) * K.kn * g.overlap + (-nrvy) * c;
+    const double fmax = sqrt(fnx * fnx + fny * fny) * K.mu;
+    const double ex = (-xix) * K.kt, ey = (-xiy) * K.kt;
+    double fsx = ex + (-srvx) * c, fsy = ey + (-srvy) * c;
+    if (sqrt(fsx * fsx + fsy * fsy) > fmax) {
+        const double ls = fsx * fsx + fsy * fsy;
+        double ux = 0.0, uy = 0.0;
+        if (ls != 0.0) { const double s = sqrt(ls); ux = fsx / s; uy = fsy / s; }
+        fsx = ux * fmax; fsy = uy * fmax;
+        if (sqrt(ex * ex + ey * ey) > fmax) { xix = (-fsx) / K.kt; xiy = (-fsy) / K.kt; }
+        else { xix = (-ex) / K.kt; xiy = (-ey) / K.kt; }
+    }
+    // src/ContactInformation.h:62-68
+    const double fx = fnx + fsx, fy = fny + fsy;
+    new_t.ct_j[e] = j;
+    new_t.ej[e] = P.eid[j];
+    new_t.xix[e] = xix; new_t.xiy[e] = xiy;
+    new_t.nx[e] = g.nx; new_t.ny[e] = g.ny;
+    new_t.c[e] = c;
+    FRec r;
+    r.fx = fx; r.fy = fy;
+    r.t1 = g.b1x * fy - g.b1y * fx;
+    r.t2 = g.b2x * (-fy) - g.b2y * (-fx);
+    new_t.fr[e] = r;
+    new_t.fnx[e] = fnx; new_t.fny[e] = fny;
+    new_t.fsx[e] = fsx; new_t.fsy[e] = fsy;
+    new_t.bx[e] = xi - xj2;
+    new_t.by[e] = yi - yj2;
+    new_t.px[e] = g.px; new_t.py[e] = g.py;
+}
+
+// One lane per work item.
+__global__ void __launch_bounds__(kBlock)
+k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t,
+          WorkList W, uint32_t* err, Counters* cnt) {
+    __shared__ int s_pre[kSeg + 1];
+    __shared__ int s_go;
+    if (threadIdx.x == 0) {
+        s_go = *err == 0;
+        segment_prefix(s_pre, W.counter, W.seg_cap);
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const int total = s_pre[kSeg];
+    unsigned long long n_newton = 0, n_rescue = 0;
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < total; w += gridDim.x * kBlock) {
+        const int lo = segment_of(s_pre, w);
+        const int2 it = W.item[lo * W.seg_cap + (w - s_pre[lo])];
+        const unsigned ux = (unsigned)it.x;
+        const int e = (int)(ux & ((1u << kLstShift) - 1u));
+        const int lst = (int)(ux >> kLstShift);
+        const int i = new_t.pr_i[e];
+        const int j = new_t.pr_j[e] & ~kExtraBit;
+        const Pack K = packs[P.pack[i]];
+        contact_item(P, F, K, old_t, new_t, e, lst, it.y, i, j, err, n_newton, n_rescue);
     }
     Counters* cs = counter_slot(cnt);
     count_add(&cs->newton, n_newton);
     count_add(&cs->rescues, n_rescue);
-    count_add(&cs->coarse, n_coarse);
 }
 
 // ---------------------------------------------------------------------------
 // collectForceAndTorque (src/BaseWorld.cpp:17-57) in the order of a
 // single-threaded reference sweep with ordered maps: contacts held by lower
 // IDs (ascending), then the particle's own contacts (partner ascending).
+// Both lists are kept in that order, so the sums are single passes; the
+// incoming index is sorted here on the step that rebuilt it.
 __global__ void __launch_bounds__(kBlock)
-k_gather(Parts P, Table t, uint32_t* err) {
+k_gather(Parts P, Table t, const int* __restrict__ rebuild, uint32_t* err) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= P.n || *err) return;
+    const size_t n = (size_t)P.n;
+    int2* refs = P.in_ref + p;                   // entry k at refs[k * n]
+    const int nin = min(P.in_cnt[p], kInCap);
+    if (*rebuild) {
+        for (int a = 1; a < nin; ++a) {
+            const int2 v = refs[a * n];
+            int b = a - 1;
+            while (b >= 0 && refs[b * n].x > v.x) {
+                refs[(b + 1) * n] = refs[b * n];
+                --b;
+            }
+            refs[(b + 1) * n] = v;
+        }
+    }
+    // a listed pair without a contact adds +0.0, which leaves a sum that
+    // starts at +0.0 unchanged
     double fx = 0.0, fy = 0.0, tq = 0.0;
     int nc = 0;
-    const int nin = min(P.in_cnt[p], kInCap);
-    const int2* refs = P.in_ref + (size_t)p * kInCap;
-    int last = -1;
     for (int k = 0; k < nin; ++k) {
-        int best = 0x7fffffff, be = -1;
-        for (int u = 0; u < nin; ++u) {
-            const int2 r = refs[u];
-            if (r.x > last && r.x < best) { best = r.x; be = r.y; }
-        }
-        if (be < 0) break;
-        last = best;
-        fx += -(t.fnx[be] + t.fsx[be]);
-        fy += -(t.fny[be] + t.fsy[be]);
-        tq += t.t2[be];
-        ++nc;
+        const int e = refs[k * n].y;
+        const bool live = t.ct_j[e] >= 0;
+        const FRec r = t.fr[e];
+        fx += live ? -r.fx : 0.0;
+        fy += live ? -r.fy : 0.0;
+        tq += live ? r.t2 : 0.0;
+        nc += live ? 1 : 0;
     }
     const int s = t.row_start[p], cnt = t.row_cnt[p];
-    last = -1;
-    for (int k = 0; k < cnt; ++k) {
-        int best = 0x7fffffff, be = -1;
-        for (int e = s; e < s + cnt; ++e) {
-            const int j = t.ct_j[e] >= 0 ? t.ej[e] : -1;
-            if (j > last && j < best) { best = j; be = e; }
-        }
-        if (be < 0) break;
-        last = best;
-        fx += t.fnx[be] + t.fsx[be];
-        fy += t.fny[be] + t.fsy[be];
-        tq += t.t1[be];
-        ++nc;
+    for (int e = s; e < s + cnt; ++e) {
+        const bool live = t.ct_j[e] >= 0;
+        const FRec r = t.fr[e];
+        fx += live ? r.fx : 0.0;
+        fy += live ? r.fy : 0.0;
+        tq += live ? r.t1 : 0.0;
+        nc += live ? 1 : 0;
     }
     P.fx[p] = fx; P.fy[p] = fy; P.tq[p] = tq; P.ncont[p] = nc;
 }
@@ -740,7 +939,8 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
 
 // Per-particle stress contribution (BaseParticle::getStressFromContactList,
 // src/BaseParticle.cpp:143-151; clumps src/GluedParticle.cpp:127-139), own
-// contacts in partner order; the host sums particles in index order.
+// contacts in partner order (the row order); the host sums particles in
+// index order.
 __global__ void __launch_bounds__(kBlock)
 k_stress(Parts P, Table t, double* out) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
@@ -754,24 +954,36 @@ k_stress(Parts P, Table t, double* out) {
             if (clump && P.up[s] != ep) break;
             const double ox = clump ? P.cbx[s] : 0.0, oy = clump ? P.cby[s] : 0.0;
             const int rs = t.row_start[s], cnt = t.row_cnt[s];
-            int last = -1;
-            for (int k = 0; k < cnt; ++k) {
-                int best = 0x7fffffff, be = -1;
-                for (int e = rs; e < rs + cnt; ++e) {
-                    const int j = t.ct_j[e] >= 0 ? t.ej[e] : -1;
-                    if (j > last && j < best) { best = j; be = e; }
-                }
-                if (be < 0) break;
-                last = best;
-                const double fx = t.fnx[be] + t.fsx[be], fy = t.fny[be] + t.fsy[be];
-                const double bx = clump ? t.bx[be] + ox : t.bx[be];
-                const double by = clump ? t.by[be] + oy : t.by[be];
+            for (int e = rs; e < rs + cnt; ++e) {
+                if (t.ct_j[e] < 0) continue;
+                const double fx = t.fr[e].fx, fy = t.fr[e].fy;
+                const double bx = clump ? t.bx[e] + ox : t.bx[e];
+                const double by = clump ? t.by[e] + oy : t.by[e];
                 sxx += fx * bx; sxy += fx * by; syx += fy * bx; syy += fy * by;
             }
         }
     }
     out[4 * (size_t)p + 0] = sxx; out[4 * (size_t)p + 1] = sxy;
     out[4 * (size_t)p + 2] = syx; out[4 * (size_t)p + 3] = syy;
+}
+
+// Fixed ellipse quantities of src/EllipticParticle.cpp:49-61, 72: b = a/AR,
+// e^2, b^2, b/a - 1, a/b - 1 (circles and clumps keep zeros).
+__global__ void k_shape(int n, const double* r, const double* ar, const int8_t* kind,
+                        double* b, double* e2, double* bsq, double* s1, double* eta) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double vb = 0.0, ve2 = 0.0, vbsq = 0.0, vs1 = 0.0, veta = 0.0;
+    if (kind[p] == KIND_ELLIPSE) {
+        const double a = r[p];
+        vb = a / ar[p];
+        const double e = sqrt(1.0 - (vb * vb) / (a * a));
+        ve2 = e * e;
+        vbsq = vb * vb;
+        vs1 = vb / a - 1.0;
+        veta = a / vb - 1.0;
+    }
+    b[p] = vb; e2[p] = ve2; bsq[p] = vbsq; s1[p] = vs1; eta[p] = veta;
 }
 
 // history restart: one contact per thread into the old table's rows
@@ -783,11 +995,12 @@ __global__ void k_load_contacts(Table t, int nc, const int* ci, const int* cj,
     if (k >= nc) return;
     const int e = slot[k];
     t.pr_i[e] = ci[k];
-    t.pr_j[e] = cj[k] | (kImgStale << kImgShift);
+    t.pr_j[e] = cj[k];
     t.ct_j[e] = cj[k];
     t.ej[e] = ej[k];
     t.xix[e] = xix[k]; t.xiy[e] = xiy[k]; t.nx[e] = nx[k]; t.ny[e] = ny[k]; t.c[e] = c[k];
-    t.fnx[e] = 0; t.fny[e] = 0; t.fsx[e] = 0; t.fsy[e] = 0; t.t1[e] = 0; t.t2[e] = 0;
+    t.fnx[e] = 0; t.fny[e] = 0; t.fsx[e] = 0; t.fsy[e] = 0;
+    t.fr[e] = FRec{0.0, 0.0, 0.0, 0.0};
     t.bx[e] = 0; t.by[e] = 0; t.px[e] = 0; t.py[e] = 0;
 }
 
@@ -820,8 +1033,10 @@ struct xdem_engine {
     // device memory
     std::vector<void*> allocs;
     Parts P{};
+    Frame F{};
     Bins B{};
     Table T[2]{};
+    WorkList W{};
     int cur = 0;                  // index of the table holding the last step
     uint32_t* err_d = nullptr;
     Counters* cnt_d = nullptr;
@@ -886,20 +1101,11 @@ int map_error_bits(xdem_engine* E, uint32_t bits) {
     return fail(E, code, m);
 }
 
-// entries in use per segment (the restart table of xdem_set_contacts keeps
-// its rows in segment 0 and no counters)
+// entries in use per segment
 int segment_counts(xdem_engine* E, const Table& t, std::vector<int>& used) {
     std::vector<int> c((size_t)kSeg * kCtrStride);
     HIPCHK(hipMemcpy(c.data(), t.counter, sizeof(int) * c.size(), hipMemcpyDeviceToHost));
     for (int s = 0; s < kSeg; ++s) used[s] = std::min(c[(size_t)s * kCtrStride], t.seg_cap);
-    if (E->steps == 0) {
-        std::vector<int> cnt(E->n);
-        HIPCHK(hipMemcpy(cnt.data(), t.row_cnt, sizeof(int) * E->n, hipMemcpyDeviceToHost));
-        long long tot = 0;
-        for (int v : cnt) tot += v;
-        used.assign(kSeg, 0);
-        used[0] = (int)tot;
-    }
     return XDEM_OK;
 }
 
@@ -919,9 +1125,10 @@ int alloc_table(xdem_engine* E, Table& t, int cap, int n) {
         (rc = dalloc(E, &t.ct_j, cap)) || (rc = dalloc(E, &t.ej, cap)))
         return rc;
     double** d[] = {&t.xix, &t.xiy, &t.nx, &t.ny, &t.c, &t.fnx, &t.fny, &t.fsx,
-                    &t.fsy, &t.t1, &t.t2, &t.bx, &t.by, &t.px, &t.py};
+                    &t.fsy, &t.bx, &t.by, &t.px, &t.py};
     for (double** q : d)
         if ((rc = dalloc(E, q, cap))) return rc;
+    if ((rc = dalloc(E, &t.fr, cap))) return rc;
     if ((rc = dalloc(E, &t.row_start, n)) || (rc = dalloc(E, &t.row_cnt, n)) ||
         (rc = dalloc(E, &t.counter, kSeg * kCtrStride)))
         return rc;
@@ -1115,8 +1322,7 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         (rc = dalloc(E, &im_d, n)) || (rc = dalloc(E, &iin_d, n)) || (rc = dalloc(E, &up_d, n)) ||
         (rc = dalloc(E, &pack_d, n)) || (rc = dalloc(E, &eid_d, n)) || (rc = dalloc(E, &kind_d, n)) ||
         (rc = dalloc(E, &flags_d, n)) || (rc = dalloc(E, &P.ncont, n)) ||
-        (rc = dalloc(E, &P.bin_of, n)) || (rc = dalloc(E, &P.gbin_of, n)) ||
-        (rc = dalloc(E, &P.rc_of, n)) || (rc = dalloc(E, &P.grc_of, n)) ||
+        (rc = dalloc(E, &P.cell, n)) ||
         (rc = dalloc(E, &P.vrc_of, n)) ||
         (rc = dalloc(E, &P.dsx, n)) || (rc = dalloc(E, &P.dsy, n)) ||
         (rc = dalloc(E, &P.in_cnt, n)) || (rc = dalloc(E, &P.in_ref, (size_t)n * kInCap)))
@@ -1155,6 +1361,25 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     }
     HIPCHK(hipMemcpy(kind_d, k8.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(flags_d, f8.data(), n, hipMemcpyHostToDevice));
+    {
+        // ellipse frame: 5 fixed and 14 per-step arrays of n doubles
+        double* fb = nullptr;
+        if ((rc = dalloc(E, &fb, (size_t)19 * n))) return rc;
+        HIPCHK(hipMemset(fb, 0, sizeof(double) * 19 * (size_t)n));
+        double* q[19];
+        for (int k = 0; k < 19; ++k) q[k] = fb + (size_t)k * n;
+        Frame& F = E->F;
+        F.b = q[0]; F.e2 = q[1]; F.bsq = q[2]; F.s1 = q[3]; F.eta = q[4];
+        F.kx = q[5]; F.ky = q[6];
+        F.txx = q[7]; F.txy = q[8]; F.tyy = q[9];
+        F.ixx = q[10]; F.ixy = q[11]; F.iyy = q[12];
+        F.axx = q[13]; F.axy = q[14]; F.ayy = q[15];
+        F.mxx = q[16]; F.mxy = q[17]; F.myy = q[18];
+        hipLaunchKernelGGL(k_shape, dim3(grid_of(n)), dim3(kBlock), 0, E->stream, n, r_d, ar_d,
+                           kind_d, q[0], q[1], q[2], q[3], q[4]);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(E->stream));
+    }
     HIPCHK(hipMemset(P.fx, 0, db));
     HIPCHK(hipMemset(P.fy, 0, db));
     HIPCHK(hipMemset(P.tq, 0, db));
@@ -1178,6 +1403,14 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     const long long tcap = std::min<long long>(12LL * n + 64LL * 4096, (1LL << kImgShift) - 1);
     if ((rc = alloc_table(E, E->T[0], (int)tcap, n)) || (rc = alloc_table(E, E->T[1], (int)tcap, n)))
         return rc;
+    // work list: a segment receives the survivors of the listed pairs that
+    // the blocks of that segment visit, at most a table segment plus one
+    // round of kBlock per block
+    E->W.seg_cap = E->T[0].seg_cap + (E->pair_grid / kSeg + 1) * kBlock;
+    if ((rc = dalloc(E, &E->W.item, (size_t)E->W.seg_cap * kSeg)) ||
+        (rc = dalloc(E, &E->W.counter, kSeg * kCtrStride)))
+        return rc;
+    HIPCHK(hipMemset(E->W.counter, 0, sizeof(int) * kSeg * kCtrStride));
     if ((rc = dalloc(E, &E->packs_d, E->npacks)) || (rc = dalloc(E, &E->err_d, 1)) ||
         (rc = dalloc(E, &E->cnt_d, kSeg)) || (rc = dalloc(E, &E->stress_d, (size_t)4 * n)) ||
         (rc = dalloc(E, &E->rebuild_d, 2)))
@@ -1212,8 +1445,12 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
     int acc = 0;
     for (int p = 0; p < E->n; ++p) { start[p] = acc; acc += cnt[p]; }
     if (acc > E->T[0].seg_cap) return fail(E, XDEM_E_CAPACITY, "too many contacts");
+    // rows in ascending partner ID, the order the step kernels keep
+    std::vector<int> order(nc);
+    for (int k = 0; k < nc; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cj[a] < cj[b]; });
     std::vector<int> fill(E->n, 0);
-    for (int k = 0; k < nc; ++k) slot[k] = start[ii[k]] + fill[ii[k]]++;
+    for (int k : order) slot[k] = start[ii[k]] + fill[ii[k]]++;
     Table& t = E->T[E->cur];
     int rc;
     int *ci_d, *cj_d, *ej_d, *slot_d;
@@ -1240,6 +1477,20 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
     }
     HIPCHK(hipMemcpyAsync(t.row_start, start.data(), sizeof(int) * E->n, hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipMemcpyAsync(t.row_cnt, cnt.data(), sizeof(int) * E->n, hipMemcpyHostToDevice, E->stream));
+    {
+        // the restored rows fill segment 0
+        std::vector<int> ctr((size_t)kSeg * kCtrStride, 0);
+        ctr[0] = acc;
+        HIPCHK(hipMemcpyAsync(t.counter, ctr.data(), sizeof(int) * ctr.size(),
+                              hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+    }
+    {
+        // the next step rebuilds the pair list around the restored history
+        const int one = 1;
+        HIPCHK(hipMemcpyAsync(E->rebuild_d + (E->steps & 1), &one, sizeof one,
+                              hipMemcpyHostToDevice, E->stream));
+    }
     HIPCHK(hipStreamSynchronize(E->stream));
     for (size_t k = base; k < E->allocs.size(); ++k) hipFree(E->allocs[k]);
     E->allocs.resize(base);
@@ -1258,8 +1509,8 @@ int launch_step(xdem_engine* E) {
     const int par = (int)(E->steps & 1);
     int* rb = E->rebuild_d + par;
     int* rb_next = E->rebuild_d + (par ^ 1);
-    hipLaunchKernelGGL(k_bin, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d, E->B,
-                       old_t, new_t, rb, rb_next, E->err_d);
+    hipLaunchKernelGGL(k_bin, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->F, E->packs_d,
+                       E->B, old_t, new_t, E->W, rb, rb_next, E->err_d);
     hipLaunchKernelGGL(k_vl_build, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
                        E->B, old_t, new_t, rb, E->err_d, E->cnt_d);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1271,12 +1522,15 @@ int launch_step(xdem_engine* E) {
         }
         e0 = E->ev_pool[E->ev_used++];
         e1 = E->ev_pool[E->ev_used++];
-        HIPCHK(hipEventRecord(e0, E->stream));
     }
-    hipLaunchKernelGGL(k_pairs, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
-                       E->packs_d, old_t, new_t, rb, E->err_d, E->cnt_d);
+    hipLaunchKernelGGL(k_coarse, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
+                       E->packs_d, old_t, new_t, E->W, rb, E->err_d, E->cnt_d);
+    if (E->timing) HIPCHK(hipEventRecord(e0, E->stream));
+    hipLaunchKernelGGL(k_contact, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P, E->F,
+                       E->packs_d, old_t, new_t, E->W, E->err_d, E->cnt_d);
     if (E->timing) HIPCHK(hipEventRecord(e1, E->stream));
-    hipLaunchKernelGGL(k_gather, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t, E->err_d);
+    hipLaunchKernelGGL(k_gather, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t, rb,
+                       E->err_d);
     const int gi = std::max(gn, grid_of(E->B.nbins) / 4 + 1);
     hipLaunchKernelGGL(k_integrate, dim3(gi), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
                        E->B, old_t.counter, rb, rb_next, E->err_d);
@@ -1403,9 +1657,16 @@ int xdem_get_contacts(xdem_engine* E, int cap, int* nc, int* ci, int* cj, double
     if ((rc = fetch(fn_x, t.fnx)) || (rc = fetch(fn_y, t.fny)) || (rc = fetch(fs_x, t.fsx)) ||
         (rc = fetch(fs_y, t.fsy)) || (rc = fetch(point_x, t.px)) || (rc = fetch(point_y, t.py)) ||
         (rc = fetch(xi_x, t.xix)) || (rc = fetch(xi_y, t.xiy)) || (rc = fetch(n_x, t.nx)) ||
-        (rc = fetch(n_y, t.ny)) || (rc = fetch(damping, t.c)) || (rc = fetch(torque_i, t.t1)) ||
-        (rc = fetch(torque_j, t.t2)))
+        (rc = fetch(n_y, t.ny)) || (rc = fetch(damping, t.c)))
         return rc;
+    if (torque_i || torque_j) {
+        std::vector<FRec> fr(used);
+        HIPCHK(hipMemcpy(fr.data(), t.fr, sizeof(FRec) * used, hipMemcpyDeviceToHost));
+        for (int k = 0; k < m; ++k) {
+            if (torque_i) torque_i[k] = fr[idx[k]].t1;
+            if (torque_j) torque_j[k] = fr[idx[k]].t2;
+        }
+    }
     return XDEM_OK;
 }
 

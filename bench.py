@@ -55,6 +55,141 @@ def make_workload(n: int, seed: int, relax_steps: int, precision: int = 0):
     return s
 
 
+def _timed(eng, steps):
+    """Wall time of `steps` steps, bracketed by device synchronisation."""
+    import torch
+    torch.cuda.synchronize()
+    eng.sync()
+    t1 = time.perf_counter()
+    eng.step(steps, sync=False)
+    eng.sync()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t1
+
+
+def _one_step_errors(s, precision, params=None):
+    """One step from the same state with the FP64 engine and with a mode:
+    pair-set agreement and relative errors of forces, torques and stress."""
+    from xdem_amd.engine import Engine
+    ref = Engine(s)
+    ref.step(1)
+    eng = Engine(s, precision=precision)
+    if params is not None:
+        eng.set_mlp(params)
+    eng.step(1)
+    cr, ce = ref.contacts(), eng.contacts()
+    same = bool(np.array_equal(cr["i"], ce["i"]) and np.array_equal(cr["j"], ce["j"]))
+    out = {"pair_set_identical": same, "contacts": int(cr["i"].size),
+           "contacts_mode": int(ce["i"].size)}
+    fr, fe = ref.forces(), eng.forces()
+    for k in ("fx", "fy", "torque"):
+        sc = max(float(np.max(np.abs(fr[k]))), 1e-300)
+        out[f"max_rel_err_{k}"] = float(np.max(np.abs(fe[k] - fr[k]))) / sc
+    sr, se = ref.stress()[0], eng.stress()[0]
+    out["max_rel_err_stress"] = float(np.max(np.abs(se - sr))) / float(np.max(np.abs(sr)))
+    ref.close()
+    eng.close()
+    return out
+
+
+def other_modes(s, args):
+    """Mixed precision and the learned root guess on the same packing."""
+    from xdem_amd.engine import Engine, PREC_MIXED, PREC_NN
+    out = {}
+    eng = Engine(s, precision=PREC_MIXED)
+    eng.step(args.warmup)
+    el = _timed(eng, args.steps)
+    c = eng.counters()
+    eng.close()
+    out["mixed"] = {"ms_per_step": el * 1e3 / args.steps,
+                    "particle_steps_per_s": s.n * args.steps / el,
+                    "newton_iters_per_pair": c["newton_iters"] / max(c["coarse_pairs"], 1),
+                    "errors_vs_fp64_one_step": _one_step_errors(s, PREC_MIXED)}
+    print("[bench] mixed done", file=sys.stderr, flush=True)
+    # learned guess: root log of the plain path during a short shear run,
+    # training with the library's kernels (untimed), then the timed run
+    t0 = time.time()
+    lg = Engine(s)
+    lg.step(args.warmup)
+    lg.mlp_init(0)
+    lg.log_roots(int(3.2e6 * args.log_steps))
+    lg.step(args.log_steps)
+    rows = lg.log_count()
+    tr, ho = lg.mlp_train(args.epochs, batch=65536, lr=2e-3, seed=0, holdout=0.02)
+    params = lg.get_mlp()
+    lg.close()
+    t_train = time.time() - t0
+    print(f"[bench] trained on {rows} rows ({t_train:.1f}s)", file=sys.stderr, flush=True)
+    eng = Engine(s, precision=PREC_NN)
+    eng.set_mlp(params)
+    eng.step(args.warmup)
+    el = _timed(eng, args.steps)
+    c = eng.counters()
+    eng.close()
+    out["nn"] = {"ms_per_step": el * 1e3 / args.steps,
+                 "particle_steps_per_s": s.n * args.steps / el,
+                 "newton_iters_per_pair": c["newton_iters"] / max(c["coarse_pairs"], 1),
+                 "fallback_rate": c["fallbacks"] / max(c["coarse_pairs"], 1),
+                 "training": {"rows": rows, "epochs": args.epochs, "seconds": t_train,
+                              "train_loss": [float(v) for v in tr],
+                              "holdout_loss": [float(v) for v in ho]},
+                 "errors_vs_fp64_one_step": _one_step_errors(s, PREC_NN, params)}
+    return out
+
+
+def ensemble_leg(args):
+    """Batched ensemble: many independent packings of uneven N in one engine
+    (one launch per phase for all of them, replayed as hipGraphs), after the
+    autotuner has picked the pair-list skin and the pair kernels' grid."""
+    from xdem_amd.autotune import autotune
+    from xdem_amd.engine import Engine
+    from xdem_amd.state import WORLD_PERIODIC, clump_branches, make_ensemble
+    t0 = time.time()
+    states = make_ensemble(args.ens_packings, seed=args.seed)
+    for frac, gamma in ((0.5, 400.0), (0.5, 40.0)):      # untimed relaxation
+        rs = []
+        for st in states:
+            r = st.copy()
+            r.cfg.world_kind = WORLD_PERIODIC
+            r.cfg.shear_rate = 0.0
+            r.cfg.mu = 0.0
+            r.cfg.global_damping = gamma
+            rs.append(r)
+        eng = Engine(rs)
+        eng.step(max(1, int(args.ens_relax * frac)))
+        p = eng.particles()
+        eng.close()
+        off = 0
+        for q, st in enumerate(states):
+            for k in ("x", "y", "theta"):
+                setattr(st, k, p[k][off:off + st.n].copy())
+            off += st.n
+            states[q] = clump_branches(st)     # branch vectors of the relaxed poses
+    n = sum(st.n for st in states)
+    t_prep = time.time() - t0
+    print(f"[bench] ensemble ready: {len(states)} packings, {n} particles ({t_prep:.1f}s)",
+          file=sys.stderr, flush=True)
+    best, trials = autotune(states, steps=30, warmup=10,
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
+    tuned = []
+    for st in states:
+        t = st.copy()
+        t.cfg.skin = best["skin_factor"] * float(np.max(np.abs(st.r)))
+        t.cfg.pair_blocks = best["pair_blocks"]
+        tuned.append(t)
+    eng = Engine(tuned)
+    eng.step(args.warmup)
+    el = _timed(eng, args.ens_steps)
+    c = eng.counters()
+    eng.close()
+    return {"workload": f"{len(states)} packings, N U[2000,50000] (total {n}), circles (AR-1 "
+                        "ellipses) + ellipses AR U[1,2.5]; half Lees-Edwards at rates "
+                        "log-U[1e-5,1e-3], half periodic with 3-disc clumps; plain FP64",
+            "particles": n, "steps": args.ens_steps, "ms_per_step": el * 1e3 / args.ens_steps,
+            "particle_steps_per_s": n * args.ens_steps / el,
+            "pair_list_builds": c["list_builds"], "autotuned": best, "autotune_trials": trials}
+
+
 def cpu_baseline(s, steps: int, threads: int):
     """The CPU oracle (FP64 restatement of the reference, OpenMP) timed on
     the host cores on the same packing."""
@@ -80,6 +215,14 @@ def main():
     ap.add_argument("--relax", type=int, default=6000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-steps", type=int, default=200)
+    ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--log-steps", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--ens-packings", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--ens-relax", type=int, default=600)
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU; multi-GPU runs "
@@ -94,22 +237,21 @@ def main():
     s = make_workload(args.n, args.seed, args.relax)
     print(f"[bench] workload ready: n={s.n} relax={args.relax} ({time.time() - t0:.1f}s)",
           file=sys.stderr, flush=True)
+    # plain FP64 path: the headline.  The timed region replays the engine's
+    # two-step hipGraphs; a second, shorter run with HIP events around every
+    # k_contact launch (eager launches) gives the kernel's own time.
     eng = Engine(s)
     eng.step(args.warmup)
-    torch.cuda.synchronize()
-    eng.set_timing(True)
-    builds0 = eng.counters()["list_builds"]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    eng.step(args.steps, sync=False)
     eng.sync()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    elapsed = t2 - t1
-    kern_ms, launches = eng.timing()
-    eng.set_timing(False)
+    builds0 = eng.counters()["list_builds"]
+    elapsed = _timed(eng, args.steps)
     bytes_step, contacts = eng.traffic()
     cnt = eng.counters()
+    eng.set_timing(True)
+    eng.step(args.kernel_steps)
+    kern_ms, launches = eng.timing()
+    eng.set_timing(False)
+    eng.close()
     ms_per_step = elapsed * 1e3 / args.steps
     value = s.n * args.steps / elapsed
     kern_avg_ms = kern_ms / max(launches, 1)
@@ -122,6 +264,13 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "k_contact_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
+    modes = {"fp64": {"ms_per_step": ms_per_step, "particle_steps_per_s": value,
+                      "newton_iters_per_pair": cnt["newton_iters"] / max(cnt["coarse_pairs"], 1)}}
+    if not args.fp64_only:
+        modes.update(other_modes(s, args))
+    for m in modes.values():
+        m["speedup_vs_fp64"] = m["particle_steps_per_s"] / value
+    ensemble = None if (args.no_ensemble or args.fp64_only) else ensemble_leg(args)
     out = {
         "metric": "particle-steps/s, 1M ellipses LE shear: plain vs NN-guess GPU; HBM % peak",
         "value": value,
@@ -148,6 +297,8 @@ def main():
                      "algorithmic_bytes_per_launch": kern_bytes,
                      "step_algorithmic_bytes": bytes_step,
                      "step_hbm_frac": bytes_step * args.steps / elapsed / (HBM_PEAK_GBS * 1e9)},
+        "modes": modes,
+        "ensemble": ensemble,
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)

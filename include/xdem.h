@@ -89,6 +89,9 @@ typedef struct xdem_config {
                                 with XDEM_E_NEWTON_CAP; 0: bisection rescue */
     double skin;             /* pair-list skin (length); <= 0: 0.2 * max outer
                                 radius.  Results do not depend on it. */
+    int    pair_blocks;      /* workgroups of the pair kernels (engine-wide,
+                                from the first packing); <= 0: 8 per CU.
+                                Results do not depend on it. */
 } xdem_config;
 
 /* Create an engine holding num_packings independent worlds. */
@@ -174,16 +177,55 @@ int xdem_get_timing(xdem_engine* e, double* contact_ms, long long* launches);
  * history read and write) and the live contact count. */
 int xdem_get_traffic(xdem_engine* e, double* bytes_per_step, long long* contacts);
 
-/* Learned quartic-root guess (XDEM_PREC_NN).  Weights of the 8-64-64-1 MLP
- * as FP32 host arrays (row-major W[out][in]); stored as BF16 on device. */
-int xdem_set_mlp(xdem_engine* e, const float* w1, const float* b1,
-                 const float* w2, const float* b2,
-                 const float* w3, const float* b3);
+/* ---- Learned quartic-root guess (XDEM_PREC_NN) --------------------------
+ * No counterpart in the reference, whose Newton iteration always starts at
+ * s = sqrt(delta + 1) (NewtonRaphsonForQuartic, src/EllipticParticle.cpp:163,
+ * 613-626).  An 8-64-64-1 ReLU network maps 8 pose/shape features of an
+ * ellipse pair (direction of the centre line and orientation of P2 in P1's
+ * frame as doubled angles, the direction of the centre line in P2's frame,
+ * b1/a1, a2/a1, b2/a1) to u = log q / log s, the place of the root q between
+ * 1 and s.  In XDEM_PREC_NN the step polishes q0 = s^u with the reference's
+ * FP64 Newton update; when the polish fails (no convergence in 16 updates, a
+ * root outside [min(1,s), max(1,s)], or a relative residual > 1e-12) the
+ * plain path runs and the fallback counter counts it.  The network runs on
+ * the matrix cores (BF16 operands, FP32 accumulation).
+ *
+ * Parameters: one FP32 vector of XDEM_MLP_PARAMS values, in this order:
+ * W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3 (W[out][in]). */
+#define XDEM_MLP_IN      8
+#define XDEM_MLP_HIDDEN  64
+#define XDEM_MLP_PARAMS  4801
 
-/* Sample (features, converged root) pairs from the next step: up to cap
- * rows of 8 features + the root + the plain Newton iteration count. */
-int xdem_log_roots(xdem_engine* e, int cap, int* count, float* features,
-                   double* roots, int* iters);
+int xdem_set_mlp(xdem_engine* e, const float* params);
+int xdem_get_mlp(xdem_engine* e, float* params);
+/* He-normal initial weights from `seed`. */
+int xdem_mlp_init(xdem_engine* e, unsigned seed);
+
+/* Root log, the training data: while enabled (cap > 0), every step appends,
+ * for each ellipse pair that passed the bounding-circle test, its features,
+ * the target u of the root the plain Newton path finds and that path's update
+ * count, up to cap rows in device memory.  cap = 0 stops and frees it. */
+int xdem_log_roots(xdem_engine* e, long long cap);
+int xdem_log_count(xdem_engine* e, long long* rows);
+/* Rows [first, first + rows): features (rows x 8, as stored: BF16 values),
+ * targets, plain Newton update counts.  Any output may be NULL. */
+int xdem_log_fetch(xdem_engine* e, long long first, int rows, float* features,
+                   float* targets, int* iters);
+
+/* Train on the root log with the library's kernels (forward and backward on
+ * the matrix cores, Adam, mean squared error of u): `epochs` passes over a
+ * seeded shuffle of the rows outside a held-out share `holdout`, mini-batches
+ * of `batch` rows.  Per-epoch mean training and hold-out losses go to the
+ * optional arrays (epochs entries each). */
+int xdem_mlp_train(xdem_engine* e, int epochs, int batch, float lr, unsigned seed,
+                   float holdout, float* train_loss, float* holdout_loss);
+
+/* Checks: one forward/backward over host rows with the current parameters
+ * (loss = mean squared error, grad = d loss / d params), and the network's
+ * output for host rows.  Features are rounded to BF16 as in the step. */
+int xdem_mlp_grad(xdem_engine* e, int rows, const float* features,
+                  const float* targets, float* loss, float* grad);
+int xdem_mlp_predict(xdem_engine* e, int rows, const float* features, float* out);
 
 #ifdef __cplusplus
 }

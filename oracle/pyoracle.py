@@ -53,6 +53,7 @@ def lib():
         L.oracle_get_boundary.argtypes = [C.c_void_p, _D]
         L.oracle_get_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_longlong)] * 5
         L.oracle_pair_geometry.argtypes = [C.c_int] + [C.c_double] * 10 + [_D]
+        L.oracle_pair_root.argtypes = [C.c_double] * 10 + [_D]
         L.oracle_newton.argtypes = [C.c_double] * 6 + [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
         L.oracle_bisect.argtypes = [C.c_double] * 6 + [C.POINTER(C.c_double)]
         _lib = L
@@ -160,6 +161,16 @@ def pair_geometry(kind, r1, ar1, x1, y1, th1, r2, ar2, x2, y2, th2):
     if rc != 0:
         raise OracleError(rc, "pair geometry")
     return out
+
+
+def pair_root(r1, ar1, x1, y1, th1, r2, ar2, x2, y2, th2):
+    """(solved, s, q, iters, (A, B, C, D, E)) of the pair's quartic (P1 = 1)
+    as the oracle's first evaluation sets it up."""
+    out = np.zeros(9)
+    rc = lib().oracle_pair_root(r1, ar1, x1, y1, th1, r2, ar2, x2, y2, th2, out)
+    if rc != 0:
+        raise OracleError(rc, "pair root")
+    return bool(out[0]), out[1], out[2], int(out[3]), tuple(out[4:9])
 
 
 def newton(A, B, Cc, D, E, init, cap=200):

@@ -65,7 +65,14 @@ struct Contact {                     // src/ContactInformation.h:15-29
     double c = -1.0;                 // _dampingConstant
 };
 
-struct Counters { long long newton = 0, coarse = 0, rescues = 0; };
+struct Counters {
+    long long newton = 0, coarse = 0, rescues = 0;
+    // the last quartic solved: start value s = sqrt(delta + 1), root, updates
+    bool solved = false;
+    double last_s = 0.0, last_q = 0.0;
+    long last_iters = 0;
+    double last_coef[5] = {0, 0, 0, 0, 0};
+};
 
 struct Geometry {                    // src/ContactGeometry.h:5-16
     double px, py, nx, ny, b1x, b1y, b2x, b2y, overlap;
@@ -319,6 +326,10 @@ static int zpm_contact(const World& W, int i, int j, double ox, double oy,
         iters = W.newton_cap;
     }
     K.newton += iters;
+    K.solved = true;
+    K.last_s = qinit; K.last_q = q; K.last_iters = iters;
+    K.last_coef[0] = A; K.last_coef[1] = B; K.last_coef[2] = C; K.last_coef[3] = D;
+    K.last_coef[4] = E;
     // eq(33)
     double temp = (q * q - 1.0) / delta;
     double dprime;
@@ -878,6 +889,34 @@ int oracle_pair_geometry(int kind, double r1, double ar1, double x1, double y1,
     out[0] = hit; out[1] = g.overlap; out[2] = g.px; out[3] = g.py;
     out[4] = g.nx; out[5] = g.ny; out[6] = g.b1x; out[7] = g.b1y;
     out[8] = g.b2x; out[9] = g.b2y; out[10] = (double)K.newton;
+    return XDEM_OK;
+}
+
+// The quartic of one ellipse pair as the reference's first evaluation sets it
+// up (P1 = 1, P2 = 2, no image offset): out = {solved (0: a transformed-frame
+// cull or the bounding-circle test rejected it), s = sqrt(delta + 1), root q,
+// Newton updates, A, B, C, D, E}.  Test infrastructure for the learned root
+// guess.
+int oracle_pair_root(double r1, double ar1, double x1, double y1, double th1, double r2,
+                     double ar2, double x2, double y2, double th2, double* out) {
+    World W;
+    W.n = 2;
+    W.kind = {KIND_ELLIPSE, KIND_ELLIPSE}; W.up = {0, 1}; W.flags = {0, 0};
+    W.r = {r1, r2}; W.ar = {ar1, ar2}; W.m = {1, 1}; W.im = {1, 1};
+    W.in = {1, 1}; W.iin = {1, 1};
+    W.x = {x1, x2}; W.y = {y1, y2}; W.th = {th1, th2};
+    W.vx = {0, 0}; W.vy = {0, 0}; W.w = {0, 0};
+    Geometry g{};
+    Counters K;
+    for (int k = 0; k < 9; ++k) out[k] = 0;
+    const double qx = x1 - x2, qy = y1 - y2;
+    if (!((r1 + r2) > std::sqrt(qx * qx + qy * qy))) return XDEM_OK;
+    const int hit = zpm_contact(W, 0, 1, 0, 0, g, K, false);
+    if (hit < 0 && !K.solved) return hit;
+    if (K.solved) {
+        out[0] = 1; out[1] = K.last_s; out[2] = K.last_q; out[3] = (double)K.last_iters;
+        for (int k = 0; k < 5; ++k) out[4 + k] = K.last_coef[k];
+    }
     return XDEM_OK;
 }
 

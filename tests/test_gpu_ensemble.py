@@ -1,0 +1,75 @@
+"""GPU: batched ensembles of uneven packings (circles as AR-1 ellipses,
+ellipses, 3-disc clumps; Lees-Edwards and periodic cells) in one engine,
+each packing against its own oracle world; and the independence of every
+result from the autotuned sizes (pair-list skin, pair-kernel grid)."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import helpers
+from xdem_amd.state import clump_branches, make_ensemble
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_ensemble():
+    out = []
+    for s in make_ensemble(6, seed=3, n_range=(300, 700)):
+        out.append(clump_branches(helpers.relax(s, 150)))
+    return out
+
+
+def test_ensemble_packings_match_oracle_worlds():
+    from xdem_amd.engine import Engine
+    states = _small_ensemble()
+    print(f"[gpu] ensemble sizes {[s.n for s in states]}, kinds "
+          f"{[int(s.cfg.world_kind) for s in states]}", flush=True)
+    eng = Engine(states)
+    eng.step(3)
+    pe, fe, se, ce = eng.particles(), eng.forces(), eng.stress(), eng.contacts()
+    for k, s in enumerate(states):
+        orc = po.OracleWorld(s)
+        orc.step(3)
+        lo, hi = eng.starts[k], eng.starts[k + 1]
+        co = orc.contacts()
+        sel = (ce["i"] >= lo) & (ce["i"] < hi)
+        assert np.array_equal(ce["i"][sel] - lo, co["i"]) and np.array_equal(ce["j"][sel] - lo, co["j"]), k
+        po_, fo = orc.particles(), orc.forces()
+        assert float(np.max(np.abs(pe["x"][lo:hi] - po_["x"]))) <= 1e-9, k
+        scale = max(float(np.max(np.abs(fo["fx"]))), 1e-300)
+        assert float(np.max(np.abs(fe["fx"][lo:hi] - fo["fx"]))) <= 1e-6 * scale, k
+        so = orc.stress()
+        assert float(np.max(np.abs(se[k] - so))) <= 1e-7 * float(np.max(np.abs(so))), k
+        assert np.allclose(eng.boundary()[k], orc.boundary(), rtol=0, atol=0), k
+
+
+def test_results_do_not_depend_on_tuned_sizes():
+    from xdem_amd.engine import Engine
+    states = _small_ensemble()
+    runs = []
+    for skin, blocks in ((0.05, 64), (0.4, 4096)):
+        tuned = []
+        for s in states:
+            t = s.copy()
+            t.cfg.skin = skin * float(np.max(np.abs(s.r)))
+            t.cfg.pair_blocks = blocks
+            tuned.append(t)
+        eng = Engine(tuned)
+        eng.step(6)
+        runs.append((eng.particles(), eng.forces(), eng.contacts(), eng.counters()))
+    (pa, fa, ca, na), (pb, fb, cb, nb) = runs
+    assert na["list_builds"] != nb["list_builds"]          # really different lists
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
+    for k in ("fx", "fy", "torque", "num_contacts"):
+        assert np.array_equal(fa[k], fb[k]), k
+    assert np.array_equal(ca["i"], cb["i"]) and np.array_equal(ca["fn_x"], cb["fn_x"])
+
+
+def test_autotune_picks_a_candidate():
+    from xdem_amd.autotune import autotune
+    states = _small_ensemble()
+    best, trials = autotune(states, skin_factors=(0.1, 0.3), blocks_per_cu=(2, 8), steps=10,
+                            warmup=3)
+    assert len(trials) == 4 and best in trials
+    assert all(np.isfinite(t["ms_per_step"]) and t["ms_per_step"] > 0 for t in trials)

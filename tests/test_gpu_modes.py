@@ -1,0 +1,209 @@
+"""GPU: the learned root guess (network kernels, root log, training, the
+learned-guess step mode with its ablation) and the mixed-precision mode.
+
+Bars:
+  * network forward: against the CPU oracle with the engine's BF16 rounding
+    points; FP32 accumulation order differs, so |dy| <= 1e-5 typical and
+    <= 2e-2 where a BF16 rounding of an activation flips;
+  * gradient: relative L2 error <= 1e-3 against the oracle, loss <= 1e-5;
+  * learned-guess mode: the FP64 bars of the plain path (test_gpu_parity),
+    contact sets identical: the guess only changes Newton's start, and only
+    where the quartic has a single root in [1, s] (delta > 0);
+  * mixed mode: contact sets identical; per-quantity errors against the FP64
+    oracle within the bounds stated in MIXED_BOUNDS (relative to each
+    field's largest magnitude).
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+from oracle import pyoracle as po
+from tests import helpers
+from tests.test_gpu_parity import assert_parity, oracle_with_envelope, _readback
+
+pytestmark = pytest.mark.gpu
+
+# stated bounds of the mixed mode, one step from a shared state: relative to
+# the field's largest magnitude (forces, torques, stress, normals, points) or
+# absolute (x, y, theta after the step).  Measured maxima over the three
+# packings are about 20x smaller.
+MIXED_BOUNDS = {"fn_x": 1e-5, "fn_y": 1e-5, "fs_x": 1e-5, "fs_y": 1e-5, "torque_i": 1e-5,
+                "torque_j": 1e-5, "fx": 1e-5, "fy": 1e-5, "torque": 1e-5, "stress": 1e-6,
+                "n_x": 1e-10, "n_y": 1e-10, "point_x": 1e-10, "point_y": 1e-10,
+                "x": 1e-9, "y": 1e-9, "theta": 1e-9}
+
+
+def _engine(s, precision=0):
+    from xdem_amd.engine import Engine
+    return Engine(s, precision=precision)
+
+
+def test_mlp_forward_matches_oracle():
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-1, 1, (1000, mo.IN)).astype(np.float32)
+    p = mo.init_params(1)
+    eng = _engine(helpers.state("pb_ellipse"))
+    eng.set_mlp(p)
+    y = eng.mlp_predict(X)
+    ref = mo.forward(p, X)["y"]
+    d = np.abs(y - ref)
+    print(f"[gpu] mlp forward: median {np.median(d):.2e} max {d.max():.2e}", flush=True)
+    assert np.median(d) <= 1e-5 and d.max() <= 2e-2
+    assert np.array_equal(eng.get_mlp(), p)
+
+
+def test_mlp_gradient_matches_oracle():
+    rng = np.random.default_rng(1)
+    X = rng.uniform(-1, 1, (777, mo.IN)).astype(np.float32)
+    T = rng.uniform(0, 1, 777).astype(np.float32)
+    p = mo.init_params(2)
+    eng = _engine(helpers.state("pb_ellipse"))
+    eng.set_mlp(p)
+    loss, g = eng.mlp_grad(X, T)
+    lref, gref = mo.loss_and_grad(p, X, T, rounding=True)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print(f"[gpu] mlp gradient: rel L2 {rel:.2e}, loss {loss:.6e} vs {lref:.6e}", flush=True)
+    assert rel <= 1e-3
+    assert abs(loss - lref) <= 1e-5 * lref
+    for name, lo, hi in (("W1", mo.OFF_W1, mo.OFF_B1), ("W2", mo.OFF_W2, mo.OFF_B2),
+                         ("w3", mo.OFF_W3, mo.OFF_B3)):
+        r = np.linalg.norm(g[lo:hi] - gref[lo:hi]) / np.linalg.norm(gref[lo:hi])
+        assert r <= 2e-3, name
+
+
+def _oracle_log_rows(s):
+    """(u, iters, features) of every ellipse pair of a periodic packing that
+    passes the bounding-circle test (minimum image) and has delta > 0, from
+    the oracle."""
+    L = s.cfg.xmax - s.cfg.xmin
+    H = s.cfg.ymax - s.cfg.ymin
+    rows = []
+    for i in range(s.n):
+        for j in range(i + 1, s.n):
+            dx, dy = s.x[j] - s.x[i], s.y[j] - s.y[i]
+            ox = -L if dx > L / 2 else (L if dx < -L / 2 else 0.0)
+            oy = -H if dy > H / 2 else (H if dy < -H / 2 else 0.0)
+            xj, yj = s.x[j] + ox, s.y[j] + oy
+            if not (s.r[i] + s.r[j] > np.hypot(s.x[i] - xj, s.y[i] - yj)):
+                continue
+            solved, sv, q, it, _ = po.pair_root(s.r[i], s.aspect[i], s.x[i], s.y[i], s.theta[i],
+                                             s.r[j], s.aspect[j], xj, yj, s.theta[j])
+            if not solved or not (sv > 1.0) or it >= 200 or q <= 0:
+                continue
+            f = mo.root_features(xj - s.x[i], yj - s.y[i], np.cos(s.theta[i]), np.sin(s.theta[i]),
+                                 np.cos(s.theta[j]), np.sin(s.theta[j]), s.r[i],
+                                 s.r[i] / s.aspect[i], s.r[j], s.r[j] / s.aspect[j])
+            rows.append((mo.target(q, sv), it, f))
+    return rows
+
+
+def test_root_log_matches_oracle():
+    s = helpers.state("pb_ellipse")
+    eng = _engine(s)
+    eng.log_roots(100000)
+    eng.step(1)
+    f, u, it = eng.log_fetch()
+    ref = _oracle_log_rows(s)
+    print(f"[gpu] root log: {len(u)} rows, oracle {len(ref)}", flush=True)
+    assert len(u) == len(ref) and len(u) > 100
+    og = np.lexsort((it, u))
+    oo = sorted(range(len(ref)), key=lambda k: (ref[k][0], ref[k][1]))
+    ru = np.array([ref[k][0] for k in oo])
+    ri = np.array([ref[k][1] for k in oo])
+    rf = np.array([ref[k][2] for k in oo])
+    assert np.allclose(u[og], ru, rtol=0, atol=1e-6)
+    assert np.array_equal(it[og], ri)
+    assert np.allclose(f[og], mo.bf16(rf), atol=2e-2)
+    eng.log_roots(0)
+    assert eng.log_count() == 0
+
+
+def _trained_engine(s, epochs=40, seed=0):
+    """Engine with a network trained on 30 steps of its own root log."""
+    eng = _engine(s)
+    eng.mlp_init(seed)
+    eng.log_roots(2_000_000)
+    eng.step(30)
+    n = eng.log_count()
+    tr, ho = eng.mlp_train(epochs, batch=512, lr=2e-3, seed=seed, holdout=0.1)
+    params = eng.get_mlp()
+    eng.log_roots(0)
+    return eng, params, n, tr, ho
+
+
+def test_training_reduces_loss():
+    s = helpers.state("le_poly")
+    eng, params, n, tr, ho = _trained_engine(s, epochs=30)
+    print(f"[gpu] training on {n} rows: loss {tr[0]:.4e} -> {tr[-1]:.4e}, "
+          f"hold-out {ho[0]:.4e} -> {ho[-1]:.4e}", flush=True)
+    assert n > 5000
+    assert np.all(np.isfinite(tr)) and np.all(np.isfinite(ho))
+    assert tr[-1] < 0.5 * tr[0] and ho[-1] < 0.5 * ho[0]
+    assert ho[-1] < 0.01
+
+
+def _nn_run(s, params):
+    eng = _engine(s, precision=2)
+    eng.set_mlp(params)
+    eng.step(1)
+    return eng
+
+
+def test_learned_guess_parity_and_ablation():
+    """The learned-guess mode gives the FP64 results (same bars as the plain
+    path) and, with the trained network, fewer Newton updates per pair than
+    the plain start value and than an untrained network."""
+    s = helpers.state("le_poly")
+    _, params, _, _, _ = _trained_engine(s, epochs=40)
+    st = po.OracleWorld(s)
+    st.step(35)
+    s2 = st.export_state(s)
+    ref, env = oracle_with_envelope(s2)
+    nn = _nn_run(s2, params)
+    assert_parity(_readback(nn), ref, env, "learned guess")
+    plain = _engine(s2)
+    plain.step(1)
+    cp, cn = plain.counters(), nn.counters()
+    untrained = _nn_run(s2, mo.init_params(9))
+    cu = untrained.counters()
+    ip = cp["newton_iters"] / cp["coarse_pairs"]
+    inn = cn["newton_iters"] / cn["coarse_pairs"]
+    iu = cu["newton_iters"] / cu["coarse_pairs"]
+    fn = cn["fallbacks"] / cn["coarse_pairs"]
+    fu = cu["fallbacks"] / cu["coarse_pairs"]
+    print(f"[gpu] Newton updates per pair: plain {ip:.3f}, trained guess {inn:.3f}, "
+          f"untrained guess {iu:.3f}; fallback rate trained {fn:.4f}, untrained {fu:.4f}",
+          flush=True)
+    assert inn < 0.8 * ip            # what the trained network gains
+    assert inn < iu and fn <= fu     # and what an untrained one does not
+    assert fn < 0.05
+
+
+@pytest.mark.parametrize("name", ["le_poly", "pb_ellipse", "pb_circle"])
+def test_mixed_precision_error_bounds(name):
+    s = helpers.history_state(name, steps=5)
+    orc = po.OracleWorld(s)
+    orc.step(1)
+    ref = _readback(orc)
+    eng = _engine(s, precision=1)
+    eng.step(1)
+    got = _readback(eng)
+    ce, co = got["contacts"], ref["contacts"]
+    assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"]), name
+    errs = {}
+    for k in ("fn_x", "fn_y", "fs_x", "fs_y", "torque_i", "torque_j", "n_x", "n_y",
+              "point_x", "point_y"):
+        sc = max(float(np.max(np.abs(co[k]))), 1e-300)
+        errs[k] = float(np.max(np.abs(ce[k] - co[k]))) / sc
+    for k in ("fx", "fy", "torque"):
+        fo = ref["forces"][k]
+        sc = max(float(np.max(np.abs(fo))), 1e-300)
+        errs[k] = float(np.max(np.abs(got["forces"][k] - fo))) / sc
+    so = ref["stress"]
+    errs["stress"] = float(np.max(np.abs(got["stress"] - so))) / float(np.max(np.abs(so)))
+    for k in ("x", "y", "theta"):
+        errs[k] = float(np.max(np.abs(got["particles"][k] - ref["particles"][k])))
+    print(f"[gpu] mixed {name}: " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()),
+          flush=True)
+    for k, v in errs.items():
+        assert v <= MIXED_BOUNDS[k], (name, k, v)

@@ -1,7 +1,7 @@
 // engine.hip -- MI355X (gfx950) DEM step engine behind the C-ABI of
 // include/xdem.h.
 //
-// One step of every packing is six launches on the engine's stream:
+// One step of every packing is seven launches on the engine's stream:
 //   k_bin         wrap positions (modifyParticlePosition), cell index, the
 //                 per-particle ellipse frame of this step; on a list-rebuild
 //                 step also fills fixed-capacity cell bins and the two ghost
@@ -10,6 +10,7 @@
 //                 reference's candidate filters; keeps pairs whose bounding
 //                 circles come within a skin; rows sorted by partner ID,
 //                 block-compacted pair list, per-particle incoming index
+//   k_in_sort     (rebuild steps only) incoming index in ID order
 //   k_coarse      one lane per listed pair: the reference's candidate rule
 //                 and image offset from this step's cells, bounding-circle
 //                 test; erases / carries history; compacts the survivors
@@ -29,11 +30,15 @@
 #include <string.h>
 #include <algorithm>
 #include <cmath>
+#include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/xdem.h"
 #include "dem_math.h"
+#include "mlp.h"
+#include "mlp_launch.h"
 
 using namespace xdem;
 
@@ -72,6 +77,12 @@ struct Frame {
     double *txx, *txy, *tyy, *ixx, *ixy, *iyy, *axx, *axy, *ayy, *mxx, *mxy, *myy;
 };
 
+// Force record of a contact, read by the per-particle sums: F = Fn + Fs on
+// P1 and the torques b1 x F, b2 x (-F) (src/ContactInformation.h:62-68).
+struct alignas(32) FRec {
+    double fx, fy, t1, t2;
+};
+
 struct Parts {
     double *x, *y, *th, *vx, *vy, *w;
     const double *r, *ar, *m, *im, *iin;
@@ -87,19 +98,14 @@ struct Parts {
     // y its Lees-Edwards ghost copy (-1 when it has none)
     int2* cell;
     int* vrc_of;                  // list grid cell of the last rebuild
-    // listed pairs held by lower IDs, per particle, as (ID of P1, slot),
-    // sorted by ID; rebuilt with the pair list
+    // Incoming index: the listed pairs held by lower IDs, per particle, as
+    // (ID of P1, slot), sorted by ID at each list rebuild; entry k of
+    // particle p at [k * n + p].
     int* in_cnt;
     int2* in_ref;
     double *dsx, *dsy;            // displacement since the list rebuild (a
                                   // path-length bound for clump subs in dsx)
     int n;
-};
-
-// Force record of a contact, read by the per-particle sums: F = Fn + Fs on
-// P1 and the torques b1 x F, b2 x (-F) (src/ContactInformation.h:62-68).
-struct alignas(32) FRec {
-    double fx, fy, t1, t2;
 };
 
 // Pair / contact table.  Rows hold a particle's listed partners j > i in
@@ -120,11 +126,35 @@ struct Table {
 // list << kLstShift), y = slot of the pair's history in the previous table
 // or -1.  Segmented like the table.
 constexpr int kLstShift = 30;
+// In the learned-guess mode each item also carries the pair's features
+// (BF16) and the network's guess.
 struct WorkList {
     int2* item;
+    uint4* feat;
+    float* guess;
     int* counter;                 // kSeg counters, kCtrStride apart
     int seg_cap;
 };
+
+// Root log for training the guess: features, target u = log q / log s and
+// the plain Newton update count of every ellipse pair solved.
+struct RootLog {
+    uint4* feat;
+    float* targ;
+    int* its;
+    unsigned long long* count;
+    long long cap;
+};
+
+// Precision modes (XDEM_PREC_*): kernels are instantiated per mode.
+constexpr int kFP64 = XDEM_PREC_FP64, kMixed = XDEM_PREC_MIXED, kNN = XDEM_PREC_NN;
+
+// FP32 cell index for the mixed mode, FP64 (the reference's) otherwise.
+template <int PREC>
+__device__ __forceinline__ int cell_index(double v, double lo, double len) {
+    if (PREC == kMixed) return (int)floorf(((float)v - (float)lo) / (float)len);
+    return (int)floor((v - lo) / len);
+}
 
 struct Bins {
     int* count;
@@ -186,6 +216,7 @@ __device__ __forceinline__ void image_offset(const Pack& K, int img, double xi, 
 // particle (src/PeriodicBoundaryWorld.cpp:32-33).  On a rebuild step the
 // cell bins and the two ghost rows are filled; otherwise the pair list
 // (rows and segment counters) carries over to the new table.
+template <int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
       WorkList W, const int* __restrict__ rebuild, int* rebuild_next, uint32_t* err) {
@@ -227,8 +258,8 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
     else if (x > K.xmax) x += -K.xlen;
     if (x != x0 || y != y0) { P.x[p] = x; P.y[p] = y; }
     if (rb) P.in_cnt[p] = 0;
-    int ix = (int)floor((x - K.xmin) / K.lenx);
-    int iy = (int)floor((y - K.ymin) / K.leny);
+    int ix = cell_index<PREC>(x, K.xmin, K.lenx);
+    int iy = cell_index<PREC>(y, K.ymin, K.leny);
     ix = min(max(ix, 0), K.nx - 1);
     iy = min(max(iy, 0), K.ny - 1);
     int grc = -1;
@@ -243,7 +274,7 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
             double t = iy == 0 ? x + K.shift : x - K.shift;
             if (t < K.xmin) t += K.xlen;
             else if (t > K.xmax) t -= K.xlen;
-            gx = (int)floor((t - K.xmin) / K.lenx);
+            gx = cell_index<PREC>(t, K.xmin, K.lenx);
             gx = min(max(gx, 0), K.nx - 1);
         }
         grc = (grow << 16) | gx;
@@ -464,6 +495,25 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     }
 }
 
+// Rebuild steps: order each particle's incoming index by the ID of P1.
+__global__ void __launch_bounds__(kBlock)
+k_in_sort(Parts P, const int* __restrict__ rebuild, uint32_t* err) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (*rebuild == 0 || p >= P.n || *err) return;
+    const size_t n = (size_t)P.n;
+    int2* refs = P.in_ref + p;                   // entry k at refs[k * n]
+    const int nin = min(P.in_cnt[p], kInCap);
+    for (int a = 1; a < nin; ++a) {
+        const int2 v = refs[a * n];
+        int b = a - 1;
+        while (b >= 0 && refs[b * n].x > v.x) {
+            refs[(b + 1) * n] = refs[b * n];
+            --b;
+        }
+        refs[(b + 1) * n] = v;
+    }
+}
+
 // Prefix of the kSeg segment counts (capped at the segment size) into
 // s_pre[0..kSeg]; thread 0 of the block.
 __device__ __forceinline__ void segment_prefix(int* s_pre, const int* ctr, int seg_cap) {
@@ -492,9 +542,15 @@ __device__ __forceinline__ int segment_of(const int* s_pre, int w) {
 // history entry whose pair is no longer a candidate is carried forward
 // untouched (src/BaseWorld.cpp:34-46); a candidate that fails the test is
 // erased (src/BaseParticle.cpp:93-97); the others go to the contact kernel
-// as block-compacted work items.
+// as block-compacted work items.  Mixed mode: the test runs in FP32 with a
+// relative margin that covers its rounding, so it passes every pair the FP64
+// test passes (the geometry then decides, in FP64).  Learned-guess mode: the
+// item also receives the pair's features.
+constexpr float kCullMargin = 1.000001f;
+
+template <int PREC>
 __global__ void __launch_bounds__(kBlock)
-k_coarse(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, WorkList W,
+k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t, WorkList W,
          const int* __restrict__ rebuild, uint32_t* err, Counters* cnt) {
     __shared__ int s_pre[kSeg + 1];
     __shared__ int s_wave[kBlock / 64];
@@ -516,6 +572,9 @@ k_coarse(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, Work
         const int w = w0 + threadIdx.x;
         bool push = false;
         int2 item = make_int2(0, -1);
+        float feat[kMlpIn];
+        if (PREC == kNN)
+            for (int k = 0; k < kMlpIn; ++k) feat[k] = 0.f;
         if (w < total) {
             const int lo = segment_of(s_pre, w);
             const int e = lo * new_t.seg_cap + (w - s_pre[lo]);
@@ -547,7 +606,12 @@ k_coarse(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, Work
                     const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
                     double ox, oy, ovx;
                     image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
-                    if (ki == KIND_CIRCLE) {
+                    if (PREC == kMixed) {
+                        const float dxf = (float)((xj + ox) - xi), dyf = (float)((yj + oy) - yi);
+                        const float rs = (float)(P.r[i] + P.r[j]) * kCullMargin;
+                        push = rs >= sqrtf(dxf * dxf + dyf * dyf);
+                        n_coarse += (ki == KIND_CIRCLE || push) ? 1 : 0;
+                    } else if (ki == KIND_CIRCLE) {
                         const double bvx = (xj + ox) - xi, bvy = (yj + oy) - yi;
                         push = !((P.r[i] + P.r[j]) - sqrt(bvx * bvx + bvy * bvy) < 0.0);
                         ++n_coarse;
@@ -555,6 +619,11 @@ k_coarse(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, Work
                         const double qx = xi - (xj + ox), qy = yi - (yj + oy);
                         push = (P.r[i] + P.r[j]) > sqrt(qx * qx + qy * qy);
                         n_coarse += push ? 1 : 0;
+                        if (PREC == kNN && push)
+                            root_features((float)((xj + ox) - xi), (float)((yj + oy) - yi),
+                                          (float)F.kx[i], (float)F.ky[i], (float)F.kx[j],
+                                          (float)F.ky[j], (float)P.r[i], (float)F.b[i],
+                                          (float)P.r[j], (float)F.b[j], feat);
                     }
                 }
                 if (push) {
@@ -603,23 +672,116 @@ k_coarse(Parts P, const Pack* __restrict__ packs, Table old_t, Table new_t, Work
             s_base = base;
         }
         __syncthreads();
-        if (push && s_base >= 0) W.item[s_base + s_wave[wave] + before] = item;
+        if (push && s_base >= 0) {
+            const int at = s_base + s_wave[wave] + before;
+            W.item[at] = item;
+            if (PREC == kNN) W.feat[at] = pack_features(feat);
+        }
         __syncthreads();
     }
     count_add(&counter_slot(cnt)->coarse, n_coarse);
 }
 
 // ---------------------------------------------------------------------------
+// Root of the ZPM quartic for one pair (Newton of
+// src/EllipticParticle.cpp:613-626 from s = sqrt(delta + 1), capped, with the
+// bisection rescue).  Mixed mode: the same Newton in FP32 to 1e-6 relative,
+// then polished by the FP64 update until it moves by <= 1e-10.  Learned-guess
+// mode, for delta > 0: polished from q0 = s^u (u the network's guess); the
+// polished root is kept when it converges within kPolishCap updates, lies in
+// [1, s] (where sin^2 psi = (q^2 - 1)/delta is in [0, 1]) and leaves a
+// relative residual <= 1e-12; otherwise the plain path runs.  With delta > 0
+// the quartic has one root in [1, s] and Newton from s converges to it, so
+// both starts give the same root (to the last bits, which the stopping rule
+// leaves open either way).  With delta < 0 (the axis-swapped poses of the
+// reference's int-abs eigenvalue order) Newton from s can settle on a root
+// outside [s, 1]; there the plain path always runs.
+// Returns false when no root could be found (cap reached, no bracket).
+constexpr int kPolishCap = 16;
+
+__device__ __forceinline__ int newton_quartic_f32(float A, float B, float C, float D, float E,
+                                                  float init, int cap, float* root) {
+    const float A0 = 4.0f * A, B0 = 3.0f * B, C0 = 2.0f * C;
+    float r0 = init, r = 0.0f;
+    int it = 0;
+    while (fabsf(r0 - r) > 1e-6f * fabsf(r0)) {
+        if (it >= cap) return -1;
+        r = r0;
+        const float num = (((A * r0 + B) * r0 + C) * r0 + D) * r0 + E;
+        const float den = ((A0 * r0 + B0) * r0 + C0) * r0 + D;
+        r0 -= num / den;
+        ++it;
+    }
+    *root = r0;
+    return it;
+}
+
+__device__ __forceinline__ bool root_accepted(const ZpmSetup& S, double s, double q) {
+    const double lo = (s < 1.0 ? s : 1.0) * (1.0 - 1e-9), hi = (s < 1.0 ? 1.0 : s) * (1.0 + 1e-9);
+    if (!(q >= lo && q <= hi)) return false;
+    const double q2 = q * q, q3 = q2 * q, q4 = q3 * q;
+    const double f = S.A * q4 + S.B * q3 + S.C * q2 + S.D * q + S.E;
+    const double sc = fabs(S.A * q4) + fabs(S.B * q3) + fabs(S.C * q2) + fabs(S.D * q) + fabs(S.E);
+    return fabs(f) <= 1e-12 * sc;
+}
+
+template <int PREC>
+__device__ __forceinline__ bool solve_root(const ZpmSetup& S, const Pack& K, float guess,
+                                           double& q, unsigned long long& n_newton,
+                                           unsigned long long& n_rescue,
+                                           unsigned long long& n_fallback) {
+    const double qinit = sqrt(S.delta + 1.0);
+    if (PREC == kNN && S.delta > 0.0) {
+        if (isfinite(guess) && fabsf(guess) < 4.0f) {
+            const double q0 = (double)expf(guess * logf((float)qinit));
+            double qq;
+            const int its = newton_quartic(S.A, S.B, S.C, S.D, S.E, q0, kPolishCap, &qq);
+            n_newton += (unsigned long long)(its < 0 ? kPolishCap : its);
+            if (its >= 0 && root_accepted(S, qinit, qq)) {
+                q = qq;
+                return true;
+            }
+        }
+        ++n_fallback;
+    } else if (PREC == kMixed) {
+        float qf;
+        if (newton_quartic_f32((float)S.A, (float)S.B, (float)S.C, (float)S.D, (float)S.E,
+                               (float)qinit, 64, &qf) >= 0 && isfinite(qf)) {
+            double qq;
+            const int its = newton_quartic(S.A, S.B, S.C, S.D, S.E, (double)qf, K.newton_cap, &qq);
+            if (its >= 0) {
+                n_newton += (unsigned long long)its;
+                q = qq;
+                return true;
+            }
+        }
+        ++n_fallback;
+    }
+    int its = newton_quartic(S.A, S.B, S.C, S.D, S.E, qinit, K.newton_cap, &q);
+    if (its < 0) {
+        if (K.strict || !quartic_bisect(S.A, S.B, S.C, S.D, S.E, qinit, &q)) return false;
+        ++n_rescue;
+        its = K.newton_cap;
+    }
+    n_newton += (unsigned long long)its;
+    return true;
+}
+
+struct StepCounts {
+    unsigned long long newton = 0, rescue = 0, fallback = 0;
+};
+
+// ---------------------------------------------------------------------------
 // One pair that passed the bounding-circle test: contact geometry
 // (EllipticParticle::contactWithEllipticParticle, src/EllipticParticle.cpp:34-285;
 // CircularParticle::contactWithCircularParticle, src/CircularParticle.cpp:67-92),
 // history and contact model (BaseParticle::calculateContactForceAndTorque,
-// src/BaseParticle.cpp:27-98).
+// src/BaseParticle.cpp:27-98).  `guess`: the network's output (learned mode).
+template <int PREC>
 __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, const Pack& K,
                                              const Table& old_t, const Table& new_t, int e,
-                                             int lst, int found, int i, int j, uint32_t* err,
-                                             unsigned long long& n_newton,
-                                             unsigned long long& n_rescue) {
+                                             int lst, int found, int i, int j, float guess,
+                                             uint32_t* err, StepCounts& n) {
     const double xi = P.x[i], yi = P.y[i];
     double ox, oy, ovx;
     image_offset(K, lst, xi, yi, P.x[j], P.y[j], ox, oy, ovx);
@@ -644,106 +806,161 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
             ZpmSetup S;
             hit = zpm_setup_frame(f, k2x, k2y, e2s, b2s, dx, dy, S, attempt == 1);
             if (!hit) break;
-            const double qinit = sqrt(S.delta + 1.0);
             double q;
-            int its = newton_quartic(S.A, S.B, S.C, S.D, S.E, qinit, K.newton_cap, &q);
-            if (its < 0) {
-                if (K.strict || !quartic_bisect(S.A, S.B, S.C, S.D, S.E, qinit, &q)) {
-                    hit = -2;
-                    break;
-                }
-                ++n_rescue;
-                its = K.newton_cap;
+            if (!solve_root<PREC>(S, K, attempt == 0 ? guess : NAN, q, n.newton, n.rescue,
+                                  n.fallback)) {
+                hit = -2;
+                break;
             }
-            n_newton += (unsigned long long)its;
             hit = zpm_finish_frame(S, q, xi, yi, xj2, yj2, dx, dy, F.ixx[i], F.ixy[i],
                                    F.iyy[i], F.axx[i], F.axy[i], F.ayy[i], g);
             if (hit == 1 && attempt == 0 &&
                 !(isfinite(g.overlap) && isfinite(g.nx) && isfinite(g.ny) &&
                   isfinite(g.px) && isfinite(g.py))) {
                 if (K.strict) { hit = -2; break; }
-                ++n_rescue;
+                ++n.rescue;
                 continue;
             }
             break;
         }
-        if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); new_t.ct_j[e] = -1; return; }
-        if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); new_t.ct_j[e] = -1; return; }
+        if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); hit = 0; }
+        if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); hit = 0; }
     }
-    if (!hit) { new_t.ct_j[e] = -1; return; }
-    // relative velocity, src/BaseParticle.cpp:11-25; LE velocities carry
-    // the background term vx + rate*y (src/LeesEdwardsBoundaryWorld.cpp:508-518)
+    if (!hit) {
+        new_t.ct_j[e] = -1;
+        return;
+    }
     const bool le = K.world == WORLD_LE;
-    const double vxi = le ? P.vx[i] + K.rate * yi : P.vx[i];
-    const double vxj = le ? P.vx[j] + K.rate * P.y[j] : P.vx[j];
-    const double wi = P.w[i], wj = P.w[j];
-    const double ax = vxi + (-wi * g.b1y), ay = P.vy[i] + wi * g.b1x;
-    const double bx = (vxj + ovx) + (-wj * g.b2y), by = (P.vy[j] + 0.0) + wj * g.b2x;
-    const double rvx = ax - bx, rvy = ay - by;
-    const double sv = rvx * g.nx + rvy * g.ny;
-    const double nrvx = g.nx * sv, nrvy = g.ny * sv;
-    const double srvx = rvx - nrvx, srvy = rvy - nrvy;
-    double xix, xiy, c;
-    if (found < 0) {
-        // new contact, src/BaseParticle.cpp:38-64
-        const double vn = rvx * g.nx + rvy * g.ny;
-        if (!isnormal(vn)) { xix = srvx * K.dt; xiy = srvy * K.dt; }
-        else {
-            const double t = g.overlap / vn;
-            const double s = t < K.dt ? t : K.dt;
-            xix = srvx * s; xiy = srvy * s;
+    double xix, xiy, c, fnx, fny, fsx, fsy, t1, t2, fx, fy;
+    if (PREC == kMixed) {
+        // contact model and history in FP32 (geometry stays FP64)
+        const float gnx = (float)g.nx, gny = (float)g.ny, gov = (float)g.overlap;
+        const float dt = (float)K.dt, kn = (float)K.kn, kt = (float)K.kt;
+        const float vxi = (float)(le ? P.vx[i] + K.rate * yi : P.vx[i]);
+        const float vxj = (float)(le ? P.vx[j] + K.rate * P.y[j] : P.vx[j]);
+        const float wi = (float)P.w[i], wj = (float)P.w[j];
+        const float b1x = (float)g.b1x, b1y = (float)g.b1y, b2x = (float)g.b2x, b2y = (float)g.b2y;
+        const float rvx = (vxi - wi * b1y) - ((vxj + (float)ovx) - wj * b2y);
+        const float rvy = ((float)P.vy[i] + wi * b1x) - ((float)P.vy[j] + wj * b2x);
+        const float sv = rvx * gnx + rvy * gny;
+        const float nrvx = gnx * sv, nrvy = gny * sv;
+        const float srvx = rvx - nrvx, srvy = rvy - nrvy;
+        float zx, zy, cf;
+        if (found < 0) {
+            if (!isnormal(sv)) { zx = srvx * dt; zy = srvy * dt; }
+            else {
+                const float t = gov / sv;
+                const float st = t < dt ? t : dt;
+                zx = srvx * st; zy = srvy * st;
+            }
+            const bool si = P.flags[i] & SKIP_TIME_INTEGRAL, sj = P.flags[j] & SKIP_TIME_INTEGRAL;
+            const float mi = (float)P.m[i], mj = (float)P.m[j];
+            const float m0 = (!si && sj) ? mi : ((si && !sj) ? mj : (mi * mj) / (mi + mj));
+            cf = sqrtf(m0) * (float)K.factor;
+        } else {
+            const float onx = (float)old_t.nx[found], ony = (float)old_t.ny[found];
+            const float ox0 = (float)old_t.xix[found], oy0 = (float)old_t.xiy[found];
+            const float dot = onx * gnx + ony * gny, det = onx * gny - ony * gnx;
+            const float h = sqrtf(dot * dot + det * det);
+            float sa = 0.f, ca = 1.f;
+            if (h != 0.f) { ca = dot / h; sa = det / h; }
+            zx = (ca * ox0 - sa * oy0) + srvx * dt;
+            zy = (sa * ox0 + ca * oy0) + srvy * dt;
+            cf = (float)old_t.c[found];
         }
-        // damping constant, src/ContactModel_LinearDissipativeFrictional.cpp:73-82
-        const bool si = P.flags[i] & SKIP_TIME_INTEGRAL, sj = P.flags[j] & SKIP_TIME_INTEGRAL;
-        if (!si && sj) c = sqrt(P.m[i]) * K.factor;
-        else if (si && !sj) c = sqrt(P.m[j]) * K.factor;
-        else {
-            const double m0 = (P.m[i] * P.m[j]) / (P.m[i] + P.m[j]);
-            c = sqrt(m0) * K.factor;
+        const float nfx = -gnx * kn * gov - nrvx * cf, nfy = -gny * kn * gov - nrvy * cf;
+        const float fmax = sqrtf(nfx * nfx + nfy * nfy) * (float)K.mu;
+        const float ex = -zx * kt, ey = -zy * kt;
+        float sfx = ex - srvx * cf, sfy = ey - srvy * cf;
+        const float ls = sfx * sfx + sfy * sfy;
+        if (sqrtf(ls) > fmax) {
+            float ux = 0.f, uy = 0.f;
+            if (ls != 0.f) { const float sq = sqrtf(ls); ux = sfx / sq; uy = sfy / sq; }
+            sfx = ux * fmax; sfy = uy * fmax;
+            if (sqrtf(ex * ex + ey * ey) > fmax) { zx = -sfx / kt; zy = -sfy / kt; }
+            else { zx = -ex / kt; zy = -ey / kt; }
         }
+        const float ffx = nfx + sfx, ffy = nfy + sfy;
+        xix = zx; xiy = zy; c = cf;
+        fnx = nfx; fny = nfy; fsx = sfx; fsy = sfy;
+        fx = ffx; fy = ffy;
+        t1 = b1x * ffy - b1y * ffx;
+        t2 = b2x * (-ffy) - b2y * (-ffx);
     } else {
-        // persisting contact, src/BaseParticle.cpp:67-82: xi turns by
-        // the angle atan2(det, dot) between the old and new normal; its
-        // cosine and sine are dot/h and det/h (h = |(dot, det)|), the same
-        // rotation to an ulp without the two transcendental calls
-        const double onx = old_t.nx[found], ony = old_t.ny[found];
-        const double ox0 = old_t.xix[found], oy0 = old_t.xiy[found];
-        const double dot = onx * g.nx + ony * g.ny;
-        const double det = onx * g.ny - ony * g.nx;
-        const double h = sqrt(dot * dot + det * det);
-        double sa = 0.0, ca = 1.0;            // atan2(0, 0) = 0
-        if (h != 0.0) { ca = dot / h; sa = det / h; }
-        const double rx = ca * ox0 + (-sa) * oy0;
-        const double ry = sa * ox0 + ca * oy0;
-        xix = rx + srvx * K.dt;
-        xiy = ry + srvy * K.dt;
-        c = old_t.c[found];
+        // relative velocity, src/BaseParticle.cpp:11-25; LE velocities carry
+        // the background term vx + rate*y (src/LeesEdwardsBoundaryWorld.cpp:508-518)
+        const double vxi = le ? P.vx[i] + K.rate * yi : P.vx[i];
+        const double vxj = le ? P.vx[j] + K.rate * P.y[j] : P.vx[j];
+        const double wi = P.w[i], wj = P.w[j];
+        const double ax = vxi + (-wi * g.b1y), ay = P.vy[i] + wi * g.b1x;
+        const double bx = (vxj + ovx) + (-wj * g.b2y), by = (P.vy[j] + 0.0) + wj * g.b2x;
+        const double rvx = ax - bx, rvy = ay - by;
+        const double sv = rvx * g.nx + rvy * g.ny;
+        const double nrvx = g.nx * sv, nrvy = g.ny * sv;
+        const double srvx = rvx - nrvx, srvy = rvy - nrvy;
+        if (found < 0) {
+            // new contact, src/BaseParticle.cpp:38-64
+            const double vn = rvx * g.nx + rvy * g.ny;
+            if (!isnormal(vn)) { xix = srvx * K.dt; xiy = srvy * K.dt; }
+            else {
+                const double t = g.overlap / vn;
+                const double st = t < K.dt ? t : K.dt;
+                xix = srvx * st; xiy = srvy * st;
+            }
+            // damping constant, src/ContactModel_LinearDissipativeFrictional.cpp:73-82
+            const bool si = P.flags[i] & SKIP_TIME_INTEGRAL, sj = P.flags[j] & SKIP_TIME_INTEGRAL;
+            if (!si && sj) c = sqrt(P.m[i]) * K.factor;
+            else if (si && !sj) c = sqrt(P.m[j]) * K.factor;
+            else {
+                const double m0 = (P.m[i] * P.m[j]) / (P.m[i] + P.m[j]);
+                c = sqrt(m0) * K.factor;
+            }
+        } else {
+            // persisting contact, src/BaseParticle.cpp:67-82: xi turns by
+            // the angle atan2(det, dot) between the old and new normal; its
+            // cosine and sine are dot/h and det/h (h = |(dot, det)|), the same
+            // rotation to an ulp without the two transcendental calls
+            const double onx = old_t.nx[found], ony = old_t.ny[found];
+            const double ox0 = old_t.xix[found], oy0 = old_t.xiy[found];
+            const double dot = onx * g.nx + ony * g.ny;
+            const double det = onx * g.ny - ony * g.nx;
+            const double h = sqrt(dot * dot + det * det);
+            double sa = 0.0, ca = 1.0;            // atan2(0, 0) = 0
+            if (h != 0.0) { ca = dot / h; sa = det / h; }
+            const double rx = ca * ox0 + (-sa) * oy0;
+            const double ry = sa * ox0 + ca * oy0;
+            xix = rx + srvx * K.dt;
+            xiy = ry + srvy * K.dt;
+            c = old_t.c[found];
+        }
+        // contact model, src/ContactModel_LinearDissipativeFrictional.cpp:3-46
+        fnx = (-g.nx) * K.kn * g.overlap + (-nrvx) * c;
+        fny = (-g.ny) * K.kn * g.overlap + (-nrvy) * c;
+        const double fmax = sqrt(fnx * fnx + fny * fny) * K.mu;
+        const double ex = (-xix) * K.kt, ey = (-xiy) * K.kt;
+        fsx = ex + (-srvx) * c;
+        fsy = ey + (-srvy) * c;
+        if (sqrt(fsx * fsx + fsy * fsy) > fmax) {
+            const double ls = fsx * fsx + fsy * fsy;
+            double ux = 0.0, uy = 0.0;
+            if (ls != 0.0) { const double sq = sqrt(ls); ux = fsx / sq; uy = fsy / sq; }
+            fsx = ux * fmax; fsy = uy * fmax;
+            if (sqrt(ex * ex + ey * ey) > fmax) { xix = (-fsx) / K.kt; xiy = (-fsy) / K.kt; }
+            else { xix = (-ex) / K.kt; xiy = (-ey) / K.kt; }
+        }
+        // src/ContactInformation.h:62-68
+        fx = fnx + fsx;
+        fy = fny + fsy;
+        t1 = g.b1x * fy - g.b1y * fx;
+        t2 = g.b2x * (-fy) - g.b2y * (-fx);
     }
-    // contact model, src/ContactModel_LinearDissipativeFrictional.cpp:3-46
-    const double fnx = (-g.nx) * K.kn * g.overlap + (-nrvx) * c;
-    const double fny = (-g.ny) * K.kn * g.overlap + (-nrvy) * c;
-    const double fmax = sqrt(fnx * fnx + fny * fny) * K.mu;
-    const double ex = (-xix) * K.kt, ey = (-xiy) * K.kt;
-    double fsx = ex + (-srvx) * c, fsy = ey + (-srvy) * c;
-    if (sqrt(fsx * fsx + fsy * fsy) > fmax) {
-        const double ls = fsx * fsx + fsy * fsy;
-        double ux = 0.0, uy = 0.0;
-        if (ls != 0.0) { const double s = sqrt(ls); ux = fsx / s; uy = fsy / s; }
-        fsx = ux * fmax; fsy = uy * fmax;
-        if (sqrt(ex * ex + ey * ey) > fmax) { xix = (-fsx) / K.kt; xiy = (-fsy) / K.kt; }
-        else { xix = (-ex) / K.kt; xiy = (-ey) / K.kt; }
-    }
-    // src/ContactInformation.h:62-68
-    const double fx = fnx + fsx, fy = fny + fsy;
     new_t.ct_j[e] = j;
     new_t.ej[e] = P.eid[j];
     new_t.xix[e] = xix; new_t.xiy[e] = xiy;
     new_t.nx[e] = g.nx; new_t.ny[e] = g.ny;
     new_t.c[e] = c;
     FRec r;
-    r.fx = fx; r.fy = fy;
-    r.t1 = g.b1x * fy - g.b1y * fx;
-    r.t2 = g.b2x * (-fy) - g.b2y * (-fx);
+    r.fx = fx; r.fy = fy; r.t1 = t1; r.t2 = t2;
     new_t.fr[e] = r;
     new_t.fnx[e] = fnx; new_t.fny[e] = fny;
     new_t.fsx[e] = fsx; new_t.fsy[e] = fsy;
@@ -753,6 +970,7 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
 }
 
 // One lane per work item.
+template <int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t,
           WorkList W, uint32_t* err, Counters* cnt) {
@@ -765,68 +983,180 @@ k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table n
     __syncthreads();
     if (!s_go) return;
     const int total = s_pre[kSeg];
-    unsigned long long n_newton = 0, n_rescue = 0;
+    StepCounts n;
     for (int w = blockIdx.x * kBlock + threadIdx.x; w < total; w += gridDim.x * kBlock) {
         const int lo = segment_of(s_pre, w);
-        const int2 it = W.item[lo * W.seg_cap + (w - s_pre[lo])];
+        const int at = lo * W.seg_cap + (w - s_pre[lo]);
+        const int2 it = W.item[at];
         const unsigned ux = (unsigned)it.x;
         const int e = (int)(ux & ((1u << kLstShift) - 1u));
         const int lst = (int)(ux >> kLstShift);
         const int i = new_t.pr_i[e];
         const int j = new_t.pr_j[e] & ~kExtraBit;
         const Pack K = packs[P.pack[i]];
-        contact_item(P, F, K, old_t, new_t, e, lst, it.y, i, j, err, n_newton, n_rescue);
+        const float guess = PREC == kNN ? W.guess[at] : 0.0f;
+        contact_item<PREC>(P, F, K, old_t, new_t, e, lst, it.y, i, j, guess, err, n);
     }
     Counters* cs = counter_slot(cnt);
-    count_add(&cs->newton, n_newton);
-    count_add(&cs->rescues, n_rescue);
+    count_add(&cs->newton, n.newton);
+    count_add(&cs->rescues, n.rescue);
+    count_add(&cs->fallbacks, n.fallback);
+}
+
+// Learned-guess mode: the network over the work list, one wavefront per 16
+// items (the guess of an item that is not an ellipse pair is not used).
+__global__ void __launch_bounds__(kBlock)
+k_guess(WorkList W, const MlpPacked* __restrict__ pk, const uint32_t* err) {
+    __shared__ int s_pre[kSeg + 1];
+    __shared__ int s_go;
+    if (threadIdx.x == 0) {
+        s_go = *err == 0;
+        segment_prefix(s_pre, W.counter, W.seg_cap);
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const int total = s_pre[kSeg];
+    const int l = threadIdx.x & 63, g = l >> 4, r = l & 15;
+    MlpRegs R;
+    mlp_load(pk, R);
+    const int ntiles = (total + 15) / 16;
+    const int nw = gridDim.x * (kBlock / 64);
+    for (int tile = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nw) {
+        const int w = tile * 16 + r;
+        int at = -1;
+        if (w < total) {
+            const int lo = segment_of(s_pre, w);
+            at = lo * W.seg_cap + (w - s_pre[lo]);
+        }
+        bf16x8 x = {};
+        if (g == 0 && at >= 0) x = unpack_features(W.feat[at]);
+        MlpLane L;
+        mlp_forward(x, R.a1, R.a2, R.b1, R.b2, R.w3, R.b3, L);
+        if (g == 0 && at >= 0) W.guess[at] = L.y;
+    }
+}
+
+// Root log (training data of the guess): the plain root of every ellipse
+// work item with delta > 0 (where the guess is used), with its features and
+// Newton update count; block-compacted.
+__global__ void __launch_bounds__(kBlock)
+k_log(Parts P, Frame F, const Pack* __restrict__ packs, Table new_t, WorkList W, RootLog Lg,
+      const uint32_t* err) {
+    __shared__ int s_pre[kSeg + 1];
+    __shared__ int s_wave[kBlock / 64];
+    __shared__ unsigned long long s_base;
+    __shared__ int s_go;
+    if (threadIdx.x == 0) {
+        s_go = *err == 0;
+        segment_prefix(s_pre, W.counter, W.seg_cap);
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const int total = s_pre[kSeg];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int w0 = blockIdx.x * kBlock; w0 < total; w0 += gridDim.x * kBlock) {
+        const int w = w0 + threadIdx.x;
+        bool keep = false;
+        float feat[kMlpIn];
+        float u = 0.f;
+        int its = 0;
+        if (w < total) {
+            const int lo = segment_of(s_pre, w);
+            const int2 it = W.item[lo * W.seg_cap + (w - s_pre[lo])];
+            const unsigned ux = (unsigned)it.x;
+            const int e = (int)(ux & ((1u << kLstShift) - 1u));
+            const int lst = (int)(ux >> kLstShift);
+            const int i = new_t.pr_i[e];
+            const int j = new_t.pr_j[e] & ~kExtraBit;
+            if (P.kind[i] == KIND_ELLIPSE) {
+                const Pack K = packs[P.pack[i]];
+                const double xi = P.x[i], yi = P.y[i];
+                double ox, oy, ovx;
+                image_offset(K, lst, xi, yi, P.x[j], P.y[j], ox, oy, ovx);
+                const double dx = (P.x[j] + ox) - xi, dy = (P.y[j] + oy) - yi;
+                ZpmFrame f;
+                f.kx = F.kx[i]; f.ky = F.ky[i];
+                f.txx = F.txx[i]; f.txy = F.txy[i]; f.tyy = F.tyy[i];
+                f.mxx = F.mxx[i]; f.mxy = F.mxy[i]; f.myy = F.myy[i];
+                f.b = F.b[i]; f.e2 = F.e2[i]; f.bsq = F.bsq[i];
+                ZpmSetup S;
+                if (zpm_setup_frame(f, F.kx[j], F.ky[j], F.e2[j], F.bsq[j], dx, dy, S)) {
+                    const double s = sqrt(S.delta + 1.0);
+                    double q;
+                    its = newton_quartic(S.A, S.B, S.C, S.D, S.E, s, K.newton_cap, &q);
+                    if (its >= 0 && S.delta > 0.0 && s > 1.0 && q > 0.0) {
+                        const double uu = log(q) / log(s);
+                        if (isfinite(uu)) {
+                            u = (float)uu;
+                            keep = true;
+                            root_features((float)dx, (float)dy, (float)f.kx, (float)f.ky,
+                                          (float)F.kx[j], (float)F.ky[j], (float)P.r[i],
+                                          (float)f.b, (float)P.r[j], (float)F.b[j], feat);
+                        }
+                    }
+                }
+            }
+        }
+        const unsigned long long m = __ballot(keep ? 1 : 0);
+        const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+        if (lane == 0) s_wave[wave] = __popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int k = 0; k < kBlock / 64; ++k) {
+                const int c = s_wave[k];
+                s_wave[k] = tot;
+                tot += c;
+            }
+            s_base = tot ? atomicAdd(Lg.count, (unsigned long long)tot) : 0ULL;
+        }
+        __syncthreads();
+        if (keep) {
+            const unsigned long long at = s_base + (unsigned long long)(s_wave[wave] + before);
+            if (at < (unsigned long long)Lg.cap) {
+                Lg.feat[at] = pack_features(feat);
+                Lg.targ[at] = u;
+                Lg.its[at] = its;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
 // collectForceAndTorque (src/BaseWorld.cpp:17-57) in the order of a
 // single-threaded reference sweep with ordered maps: contacts held by lower
 // IDs (ascending), then the particle's own contacts (partner ascending).
-// Both lists are kept in that order, so the sums are single passes; the
-// incoming index is sorted here on the step that rebuilt it.
+// Both lists are kept in that order, so the sums are single passes over
+// the incoming entries and the row.
+// Mixed mode: the sums run in FP32.
+template <int PREC>
 __global__ void __launch_bounds__(kBlock)
-k_gather(Parts P, Table t, const int* __restrict__ rebuild, uint32_t* err) {
+k_gather(Parts P, Table t, uint32_t* err) {
+    typedef typename std::conditional<PREC == kMixed, float, double>::type acc_t;
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= P.n || *err) return;
     const size_t n = (size_t)P.n;
-    int2* refs = P.in_ref + p;                   // entry k at refs[k * n]
     const int nin = min(P.in_cnt[p], kInCap);
-    if (*rebuild) {
-        for (int a = 1; a < nin; ++a) {
-            const int2 v = refs[a * n];
-            int b = a - 1;
-            while (b >= 0 && refs[b * n].x > v.x) {
-                refs[(b + 1) * n] = refs[b * n];
-                --b;
-            }
-            refs[(b + 1) * n] = v;
-        }
-    }
-    // a listed pair without a contact adds +0.0, which leaves a sum that
-    // starts at +0.0 unchanged
-    double fx = 0.0, fy = 0.0, tq = 0.0;
+    acc_t fx = 0, fy = 0, tq = 0;
     int nc = 0;
     for (int k = 0; k < nin; ++k) {
-        const int e = refs[k * n].y;
-        const bool live = t.ct_j[e] >= 0;
+        const int e = P.in_ref[k * n + p].y;
+        if (t.ct_j[e] < 0) continue;
         const FRec r = t.fr[e];
-        fx += live ? -r.fx : 0.0;
-        fy += live ? -r.fy : 0.0;
-        tq += live ? r.t2 : 0.0;
-        nc += live ? 1 : 0;
+        fx += -(acc_t)r.fx;
+        fy += -(acc_t)r.fy;
+        tq += (acc_t)r.t2;
+        ++nc;
     }
     const int s = t.row_start[p], cnt = t.row_cnt[p];
     for (int e = s; e < s + cnt; ++e) {
-        const bool live = t.ct_j[e] >= 0;
+        if (t.ct_j[e] < 0) continue;
         const FRec r = t.fr[e];
-        fx += live ? r.fx : 0.0;
-        fy += live ? r.fy : 0.0;
-        tq += live ? r.t1 : 0.0;
-        nc += live ? 1 : 0;
+        fx += (acc_t)r.fx;
+        fy += (acc_t)r.fy;
+        tq += (acc_t)r.t1;
+        ++nc;
     }
     P.fx[p] = fx; P.fy[p] = fy; P.tq[p] = tq; P.ncont[p] = nc;
 }
@@ -1054,6 +1384,20 @@ struct xdem_engine {
     bool use_graphs = true;
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
+    int prec = XDEM_PREC_FP64;
+    // root-guess network: FP32 parameters, Adam moments, packed operands
+    float* mlp_prm = nullptr;
+    float* mlp_m1 = nullptr;
+    float* mlp_m2 = nullptr;
+    float* mlp_partial = nullptr;
+    size_t mlp_partial_n = 0;
+    float* mlp_scratch = nullptr;     // gradient + squared-error sum
+    xdem::MlpPacked* mlp_pk = nullptr;
+    bool mlp_ready = false;
+    long long adam_t = 0;
+    // root log
+    RootLog log{};
+    bool logging = false;
 };
 
 namespace {
@@ -1188,7 +1532,20 @@ int xdem_create(xdem_engine** out, int num_packings, const xdem_config* cfgs) {
     }
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, E->device);
-    E->pair_grid = cus * 8;
+    E->pair_grid = cfgs[0].pair_blocks > 0 ? cfgs[0].pair_blocks : cus * 8;
+    E->prec = cfgs[0].precision;
+    {
+        // network state (used by the learned-guess mode and the training calls)
+        int rc;
+        if ((rc = dalloc(E, &E->mlp_prm, kMlpParams)) || (rc = dalloc(E, &E->mlp_m1, kMlpParams)) ||
+            (rc = dalloc(E, &E->mlp_m2, kMlpParams)) ||
+            (rc = dalloc(E, &E->mlp_scratch, kMlpParams + 1)) ||
+            (rc = dalloc(E, &E->mlp_pk, 1)) || (rc = dalloc(E, &E->log.count, 1))) {
+            xdem_destroy(E);
+            return rc;
+        }
+        hipMemset(E->log.count, 0, sizeof(unsigned long long));
+    }
     *out = E;
     return XDEM_OK;
 }
@@ -1201,6 +1558,10 @@ void xdem_destroy(xdem_engine* E) {
         if (E->graph[k]) hipGraphDestroy(E->graph[k]);
     }
     for (void* p : E->allocs) hipFree(p);
+    if (E->log.feat) hipFree(E->log.feat);
+    if (E->mlp_partial) hipFree(E->mlp_partial);
+    if (E->log.targ) hipFree(E->log.targ);
+    if (E->log.its) hipFree(E->log.its);
     for (hipEvent_t ev : E->ev_pool) hipEventDestroy(ev);
     if (E->stream) hipStreamDestroy(E->stream);
     delete E;
@@ -1411,6 +1772,10 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         (rc = dalloc(E, &E->W.counter, kSeg * kCtrStride)))
         return rc;
     HIPCHK(hipMemset(E->W.counter, 0, sizeof(int) * kSeg * kCtrStride));
+    if (E->prec == XDEM_PREC_NN &&
+        ((rc = dalloc(E, &E->W.feat, (size_t)E->W.seg_cap * kSeg)) ||
+         (rc = dalloc(E, &E->W.guess, (size_t)E->W.seg_cap * kSeg))))
+        return rc;
     if ((rc = dalloc(E, &E->packs_d, E->npacks)) || (rc = dalloc(E, &E->err_d, 1)) ||
         (rc = dalloc(E, &E->cnt_d, kSeg)) || (rc = dalloc(E, &E->stress_d, (size_t)4 * n)) ||
         (rc = dalloc(E, &E->rebuild_d, 2)))
@@ -1502,17 +1867,19 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
 namespace {
 
 // Enqueue one step of every packing on the engine's stream.
-int launch_step(xdem_engine* E) {
+template <int PREC>
+int launch_step_t(xdem_engine* E) {
     const int gn = grid_of(E->n);
     Table& old_t = E->T[E->cur];
     Table& new_t = E->T[E->cur ^ 1];
     const int par = (int)(E->steps & 1);
     int* rb = E->rebuild_d + par;
     int* rb_next = E->rebuild_d + (par ^ 1);
-    hipLaunchKernelGGL(k_bin, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->F, E->packs_d,
-                       E->B, old_t, new_t, E->W, rb, rb_next, E->err_d);
+    hipLaunchKernelGGL(k_bin<PREC>, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->F,
+                       E->packs_d, E->B, old_t, new_t, E->W, rb, rb_next, E->err_d);
     hipLaunchKernelGGL(k_vl_build, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
                        E->B, old_t, new_t, rb, E->err_d, E->cnt_d);
+    hipLaunchKernelGGL(k_in_sort, dim3(gn), dim3(kBlock), 0, E->stream, E->P, rb, E->err_d);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (E->timing) {
         while (E->ev_pool.size() < E->ev_used + 2) {
@@ -1523,13 +1890,19 @@ int launch_step(xdem_engine* E) {
         e0 = E->ev_pool[E->ev_used++];
         e1 = E->ev_pool[E->ev_used++];
     }
-    hipLaunchKernelGGL(k_coarse, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
-                       E->packs_d, old_t, new_t, E->W, rb, E->err_d, E->cnt_d);
+    hipLaunchKernelGGL(k_coarse<PREC>, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
+                       E->F, E->packs_d, old_t, new_t, E->W, rb, E->err_d, E->cnt_d);
+    if (PREC == kNN)
+        hipLaunchKernelGGL(k_guess, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->W,
+                           E->mlp_pk, E->err_d);
     if (E->timing) HIPCHK(hipEventRecord(e0, E->stream));
-    hipLaunchKernelGGL(k_contact, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P, E->F,
-                       E->packs_d, old_t, new_t, E->W, E->err_d, E->cnt_d);
+    hipLaunchKernelGGL(k_contact<PREC>, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
+                       E->F, E->packs_d, old_t, new_t, E->W, E->err_d, E->cnt_d);
     if (E->timing) HIPCHK(hipEventRecord(e1, E->stream));
-    hipLaunchKernelGGL(k_gather, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t, rb,
+    if (E->logging)
+        hipLaunchKernelGGL(k_log, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P, E->F,
+                           E->packs_d, new_t, E->W, E->log, E->err_d);
+    hipLaunchKernelGGL(k_gather<PREC>, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t,
                        E->err_d);
     const int gi = std::max(gn, grid_of(E->B.nbins) / 4 + 1);
     hipLaunchKernelGGL(k_integrate, dim3(gi), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
@@ -1538,6 +1911,12 @@ int launch_step(xdem_engine* E) {
     E->cur ^= 1;
     E->steps++;
     return XDEM_OK;
+}
+
+int launch_step(xdem_engine* E) {
+    if (E->prec == kMixed) return launch_step_t<kMixed>(E);
+    if (E->prec == kNN) return launch_step_t<kNN>(E);
+    return launch_step_t<kFP64>(E);
 }
 
 // Capture two consecutive steps (starting at the current parity) as a graph.
@@ -1566,9 +1945,11 @@ extern "C" {
 int xdem_step(xdem_engine* E, int nsteps) {
     if (!E || !E->have_particles) return fail(E, XDEM_E_INVALID, "set particles first");
     if (nsteps < 0) return fail(E, XDEM_E_INVALID, "nsteps < 0");
+    if (E->prec == XDEM_PREC_NN && !E->mlp_ready)
+        return fail(E, XDEM_E_INVALID, "learned-guess mode: set or train the network first");
     int left = nsteps;
     int rc;
-    while (left >= 2 && E->use_graphs && !E->timing) {
+    while (left >= 2 && E->use_graphs && !E->timing && !E->logging) {
         const int key = (int)(E->steps & 1);
         if (!E->graph_exec[key] && (rc = capture_pair(E))) return rc;
         HIPCHK(hipGraphLaunch(E->graph_exec[key], E->stream));
@@ -1762,14 +2143,247 @@ int xdem_get_traffic(xdem_engine* E, double* bytes_per_step, long long* contacts
     return XDEM_OK;
 }
 
-int xdem_set_mlp(xdem_engine* E, const float*, const float*, const float*, const float*,
-                 const float*, const float*) {
-    return fail(E, XDEM_E_INVALID, "learned root guess not available in this build");
+int xdem_set_mlp(xdem_engine* E, const float* params) {
+    if (!E || !params) return fail(E, XDEM_E_INVALID, "null argument");
+    for (int k = 0; k < kMlpParams; ++k)
+        if (!std::isfinite(params[k])) return fail(E, XDEM_E_INVALID, "non-finite network parameter");
+    HIPCHK(hipStreamSynchronize(E->stream));
+    HIPCHK(hipMemcpy(E->mlp_prm, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(E->mlp_m1, 0, sizeof(float) * kMlpParams));
+    HIPCHK(hipMemset(E->mlp_m2, 0, sizeof(float) * kMlpParams));
+    E->adam_t = 0;
+    HIPCHK(mlp_pack(E->mlp_prm, E->mlp_pk, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    E->mlp_ready = true;
+    return XDEM_OK;
 }
 
-int xdem_log_roots(xdem_engine* E, int, int* count, float*, double*, int*) {
-    if (count) *count = 0;
-    return fail(E, XDEM_E_INVALID, "root logging not available in this build");
+int xdem_get_mlp(xdem_engine* E, float* params) {
+    if (!E || !params) return fail(E, XDEM_E_INVALID, "null argument");
+    if (!E->mlp_ready) return fail(E, XDEM_E_INVALID, "no network set");
+    HIPCHK(hipStreamSynchronize(E->stream));
+    HIPCHK(hipMemcpy(params, E->mlp_prm, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    return XDEM_OK;
+}
+
+int xdem_mlp_init(xdem_engine* E, unsigned seed) {
+    if (!E) return XDEM_E_INVALID;
+    // He-normal weights (ReLU layers), zero biases, output bias at the middle
+    // of the target range
+    std::mt19937 rng(seed);
+    std::vector<float> prm(kMlpParams, 0.0f);
+    std::normal_distribution<float> n1(0.0f, std::sqrt(2.0f / kMlpIn));
+    std::normal_distribution<float> n2(0.0f, std::sqrt(2.0f / kMlpHid));
+    std::normal_distribution<float> n3(0.0f, std::sqrt(1.0f / kMlpHid));
+    for (int k = 0; k < kMlpHid * kMlpIn; ++k) prm[kOffW1 + k] = n1(rng);
+    for (int k = 0; k < kMlpHid * kMlpHid; ++k) prm[kOffW2 + k] = n2(rng);
+    for (int k = 0; k < kMlpHid; ++k) prm[kOffW3 + k] = n3(rng);
+    prm[kOffB3] = 0.5f;
+    return xdem_set_mlp(E, prm.data());
+}
+
+int xdem_log_roots(xdem_engine* E, long long cap) {
+    if (!E || cap < 0) return fail(E, XDEM_E_INVALID, "bad log capacity");
+    HIPCHK(hipStreamSynchronize(E->stream));
+    if (E->log.feat) { HIPCHK(hipFree(E->log.feat)); E->log.feat = nullptr; }
+    if (E->log.targ) { HIPCHK(hipFree(E->log.targ)); E->log.targ = nullptr; }
+    if (E->log.its) { HIPCHK(hipFree(E->log.its)); E->log.its = nullptr; }
+    E->log.cap = 0;
+    E->logging = false;
+    HIPCHK(hipMemset(E->log.count, 0, sizeof(unsigned long long)));
+    if (cap == 0) return XDEM_OK;
+    HIPCHK(hipMalloc(&E->log.feat, sizeof(uint4) * (size_t)cap));
+    HIPCHK(hipMalloc(&E->log.targ, sizeof(float) * (size_t)cap));
+    HIPCHK(hipMalloc(&E->log.its, sizeof(int) * (size_t)cap));
+    E->log.cap = cap;
+    E->logging = true;
+    return XDEM_OK;
+}
+
+int xdem_log_count(xdem_engine* E, long long* rows) {
+    if (!E || !rows) return XDEM_E_INVALID;
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    unsigned long long c = 0;
+    HIPCHK(hipMemcpy(&c, E->log.count, sizeof c, hipMemcpyDeviceToHost));
+    *rows = (long long)std::min<unsigned long long>(c, (unsigned long long)E->log.cap);
+    return XDEM_OK;
+}
+
+int xdem_log_fetch(xdem_engine* E, long long first, int rows, float* features, float* targets,
+                   int* iters) {
+    long long have = 0;
+    int rc = xdem_log_count(E, &have);
+    if (rc) return rc;
+    if (first < 0 || rows < 0 || first + rows > have)
+        return fail(E, XDEM_E_INVALID, "log rows out of range");
+    if (rows == 0) return XDEM_OK;
+    if (features) {
+        std::vector<uint4> f(rows);
+        HIPCHK(hipMemcpy(f.data(), E->log.feat + first, sizeof(uint4) * rows, hipMemcpyDeviceToHost));
+        for (int k = 0; k < rows; ++k) {
+            const uint32_t w[4] = {f[k].x, f[k].y, f[k].z, f[k].w};
+            for (int q = 0; q < kMlpIn; ++q) {
+                const uint32_t h = (w[q / 2] >> (16 * (q % 2))) & 0xffffu;
+                const uint32_t bits = h << 16;
+                float v;
+                memcpy(&v, &bits, sizeof v);
+                features[(size_t)k * kMlpIn + q] = v;
+            }
+        }
+    }
+    if (targets) HIPCHK(hipMemcpy(targets, E->log.targ + first, sizeof(float) * rows, hipMemcpyDeviceToHost));
+    if (iters) HIPCHK(hipMemcpy(iters, E->log.its + first, sizeof(int) * rows, hipMemcpyDeviceToHost));
+    return XDEM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// host FP32 features -> device BF16 rows (round to nearest even)
+std::vector<uint4> pack_rows(const float* f, int rows) {
+    std::vector<uint4> out(rows);
+    for (int k = 0; k < rows; ++k) {
+        uint32_t w[4] = {0, 0, 0, 0};
+        for (int q = 0; q < kMlpIn; ++q) {
+            uint32_t b;
+            memcpy(&b, &f[(size_t)k * kMlpIn + q], sizeof b);
+            uint32_t h;
+            if ((b & 0x7fffffffu) > 0x7f800000u) h = (b >> 16) | 0x40u;      // NaN stays NaN
+            else h = (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+            w[q / 2] |= (h & 0xffffu) << (16 * (q % 2));
+        }
+        out[k] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    return out;
+}
+
+int ensure_partial(xdem_engine* E, int rows) {
+    const size_t need = mlp_partial_floats(mlp_train_blocks(rows));
+    if (need <= E->mlp_partial_n) return XDEM_OK;
+    if (E->mlp_partial) HIPCHK(hipFree(E->mlp_partial));
+    E->mlp_partial = nullptr;
+    HIPCHK(hipMalloc(&E->mlp_partial, sizeof(float) * need));
+    E->mlp_partial_n = need;
+    return XDEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xdem_mlp_grad(xdem_engine* E, int rows, const float* features, const float* targets,
+                  float* loss, float* grad) {
+    if (!E || rows <= 0 || !features || !targets) return fail(E, XDEM_E_INVALID, "bad arguments");
+    if (!E->mlp_ready) return fail(E, XDEM_E_INVALID, "no network set");
+    int rc;
+    if ((rc = ensure_partial(E, rows))) return rc;
+    std::vector<uint4> f = pack_rows(features, rows);
+    uint4* fd = nullptr;
+    float* td = nullptr;
+    HIPCHK(hipMalloc(&fd, sizeof(uint4) * rows));
+    HIPCHK(hipMalloc(&td, sizeof(float) * rows));
+    HIPCHK(hipMemcpy(fd, f.data(), sizeof(uint4) * rows, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(td, targets, sizeof(float) * rows, hipMemcpyHostToDevice));
+    AdamStep a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+    HIPCHK(mlp_train_step(fd, td, nullptr, rows, 2.0f / rows, E->mlp_pk, E->mlp_partial,
+                          E->mlp_prm, E->mlp_m1, E->mlp_m2, a, E->mlp_scratch,
+                          E->mlp_scratch + kMlpParams, E->stream));
+    std::vector<float> out(kMlpParams + 1);
+    HIPCHK(hipMemcpyAsync(out.data(), E->mlp_scratch, sizeof(float) * out.size(),
+                          hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    HIPCHK(hipFree(fd));
+    HIPCHK(hipFree(td));
+    if (grad) memcpy(grad, out.data(), sizeof(float) * kMlpParams);
+    if (loss) *loss = out[kMlpParams] / rows;
+    return XDEM_OK;
+}
+
+int xdem_mlp_predict(xdem_engine* E, int rows, const float* features, float* out) {
+    if (!E || rows <= 0 || !features || !out) return fail(E, XDEM_E_INVALID, "bad arguments");
+    if (!E->mlp_ready) return fail(E, XDEM_E_INVALID, "no network set");
+    std::vector<uint4> f = pack_rows(features, rows);
+    uint4* fd = nullptr;
+    float* od = nullptr;
+    HIPCHK(hipMalloc(&fd, sizeof(uint4) * rows));
+    HIPCHK(hipMalloc(&od, sizeof(float) * rows));
+    HIPCHK(hipMemcpy(fd, f.data(), sizeof(uint4) * rows, hipMemcpyHostToDevice));
+    HIPCHK(mlp_rows(fd, rows, E->mlp_pk, od, E->stream));
+    HIPCHK(hipMemcpyAsync(out, od, sizeof(float) * rows, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    HIPCHK(hipFree(fd));
+    HIPCHK(hipFree(od));
+    return XDEM_OK;
+}
+
+int xdem_mlp_train(xdem_engine* E, int epochs, int batch, float lr, unsigned seed,
+                   float holdout, float* train_loss, float* holdout_loss) {
+    if (!E || epochs <= 0 || batch < 16 || !(lr > 0.f) || !(holdout >= 0.f && holdout < 1.f))
+        return fail(E, XDEM_E_INVALID, "bad training arguments");
+    if (!E->mlp_ready) return fail(E, XDEM_E_INVALID, "no network set (xdem_mlp_init)");
+    long long rows = 0;
+    int rc = xdem_log_count(E, &rows);
+    if (rc) return rc;
+    if (rows < 2 * batch || rows > 0x7fffffff) return fail(E, XDEM_E_INVALID, "root log too small or too large");
+    // seeded split: the last `holdout` share of a permutation is held out
+    std::mt19937 rng(seed);
+    std::vector<int> perm((size_t)rows);
+    for (long long k = 0; k < rows; ++k) perm[(size_t)k] = (int)k;
+    std::shuffle(perm.begin(), perm.end(), rng);
+    const long long nhold = (long long)(holdout * (double)rows);
+    const long long ntrain = rows - nhold;
+    int* idx = nullptr;
+    HIPCHK(hipMalloc(&idx, sizeof(int) * (size_t)rows));
+    if ((rc = ensure_partial(E, batch)) || (rc = ensure_partial(E, (int)std::min<long long>(nhold, 1 << 22)))) {
+        hipFree(idx);
+        return rc;
+    }
+    HIPCHK(hipMemcpy(idx + ntrain, perm.data() + ntrain, sizeof(int) * (size_t)nhold, hipMemcpyHostToDevice));
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    float sq = 0.f;
+    for (int ep = 0; ep < epochs; ++ep) {
+        std::shuffle(perm.begin(), perm.begin() + ntrain, rng);
+        HIPCHK(hipMemcpy(idx, perm.data(), sizeof(int) * (size_t)ntrain, hipMemcpyHostToDevice));
+        double sum = 0.0;
+        for (long long b0 = 0; b0 + batch <= ntrain; b0 += batch) {
+            ++E->adam_t;
+            AdamStep a;
+            a.lr = lr; a.beta1 = beta1; a.beta2 = beta2; a.eps = eps;
+            a.c1 = 1.0f / (1.0f - std::pow(beta1, (float)E->adam_t));
+            a.c2 = 1.0f / (1.0f - std::pow(beta2, (float)E->adam_t));
+            a.update = 1;
+            HIPCHK(mlp_train_step(E->log.feat, E->log.targ, idx + b0, batch, 2.0f / batch,
+                                  E->mlp_pk, E->mlp_partial, E->mlp_prm, E->mlp_m1, E->mlp_m2,
+                                  a, nullptr, E->mlp_scratch + kMlpParams, E->stream));
+            HIPCHK(mlp_pack(E->mlp_prm, E->mlp_pk, E->stream));
+            HIPCHK(hipMemcpyAsync(&sq, E->mlp_scratch + kMlpParams, sizeof sq,
+                                  hipMemcpyDeviceToHost, E->stream));
+            HIPCHK(hipStreamSynchronize(E->stream));
+            sum += sq;
+        }
+        if (train_loss) train_loss[ep] = (float)(sum / (double)((ntrain / batch) * batch));
+        if (holdout_loss) {
+            double hs = 0.0;
+            const long long chunk = 1 << 22;
+            for (long long h0 = 0; h0 < nhold; h0 += chunk) {
+                const int nr = (int)std::min<long long>(chunk, nhold - h0);
+                AdamStep a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+                HIPCHK(mlp_train_step(E->log.feat, E->log.targ, idx + ntrain + h0, nr, 0.0f,
+                                      E->mlp_pk, E->mlp_partial, E->mlp_prm, E->mlp_m1,
+                                      E->mlp_m2, a, nullptr, E->mlp_scratch + kMlpParams,
+                                      E->stream));
+                HIPCHK(hipMemcpyAsync(&sq, E->mlp_scratch + kMlpParams, sizeof sq,
+                                      hipMemcpyDeviceToHost, E->stream));
+                HIPCHK(hipStreamSynchronize(E->stream));
+                hs += sq;
+            }
+            holdout_loss[ep] = nhold > 0 ? (float)(hs / (double)nhold) : 0.0f;
+        }
+    }
+    HIPCHK(hipFree(idx));
+    return XDEM_OK;
 }
 
 }  // extern "C"

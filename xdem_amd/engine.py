@@ -27,7 +27,12 @@ SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles
            "xdem_set_contacts", "xdem_step", "xdem_sync", "xdem_get_particles",
            "xdem_get_forces", "xdem_get_contacts", "xdem_get_stress", "xdem_get_boundary",
            "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
-           "xdem_set_mlp", "xdem_log_roots")
+           "xdem_set_mlp", "xdem_get_mlp", "xdem_mlp_init", "xdem_log_roots",
+           "xdem_log_count", "xdem_log_fetch", "xdem_mlp_train", "xdem_mlp_grad",
+           "xdem_mlp_predict")
+
+PREC_FP64, PREC_MIXED, PREC_NN = 0, 1, 2
+MLP_IN, MLP_HIDDEN, MLP_PARAMS = 8, 64, 4801
 
 
 class XdemConfig(C.Structure):
@@ -35,14 +40,15 @@ class XdemConfig(C.Structure):
         (k, C.c_double) for k in ("xmin", "xmax", "ymin", "ymax", "dt", "kn", "kt",
                                   "restitution", "mu", "global_damping", "shear_rate",
                                   "total_shift", "total_strain", "cell_size")] + [
-        ("newton_cap", C.c_int), ("strict_newton", C.c_int), ("skin", C.c_double)]
+        ("newton_cap", C.c_int), ("strict_newton", C.c_int), ("skin", C.c_double),
+        ("pair_blocks", C.c_int)]
 
     @classmethod
     def from_world(cls, c: WorldConfig, precision: int = 0) -> "XdemConfig":
         return cls(c.world_kind, precision, c.xmin, c.xmax, c.ymin, c.ymax, c.dt, c.kn,
                    c.kt, c.restitution, c.mu, c.global_damping, c.shear_rate,
                    c.total_shift, c.total_strain, c.cell_size, c.newton_cap,
-                   c.strict_newton, getattr(c, "skin", 0.0))
+                   c.strict_newton, getattr(c, "skin", 0.0), getattr(c, "pair_blocks", 0))
 
 
 _D = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -78,8 +84,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_set_timing.argtypes = [vp, C.c_int]
     L.xdem_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.xdem_get_traffic.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
-    L.xdem_set_mlp.argtypes = [vp] + [_F] * 6
-    L.xdem_log_roots.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _F, _D, _I]
+    L.xdem_set_mlp.argtypes = [vp, _F]
+    L.xdem_get_mlp.argtypes = [vp, _F]
+    L.xdem_mlp_init.argtypes = [vp, C.c_uint]
+    L.xdem_log_roots.argtypes = [vp, C.c_longlong]
+    L.xdem_log_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.xdem_log_fetch.argtypes = [vp, C.c_longlong, C.c_int, vp, vp, vp]
+    L.xdem_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint, C.c_float, vp, vp]
+    L.xdem_mlp_grad.argtypes = [vp, C.c_int, _F, _F, C.POINTER(C.c_float), _F]
+    L.xdem_mlp_predict.argtypes = [vp, C.c_int, _F, _F]
     _lib = L
     return L
 
@@ -216,3 +229,58 @@ class Engine:
         b, c = C.c_double(0), C.c_longlong(0)
         self._check(self.L.xdem_get_traffic(self.h, C.byref(b), C.byref(c)))
         return b.value, c.value
+
+    # -- learned root guess -------------------------------------------------
+    def set_mlp(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_PARAMS
+        self._check(self.L.xdem_set_mlp(self.h, p))
+
+    def get_mlp(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self._check(self.L.xdem_get_mlp(self.h, p))
+        return p
+
+    def mlp_init(self, seed: int = 0):
+        self._check(self.L.xdem_mlp_init(self.h, int(seed)))
+
+    def log_roots(self, cap: int):
+        self._check(self.L.xdem_log_roots(self.h, int(cap)))
+
+    def log_count(self) -> int:
+        n = C.c_longlong(0)
+        self._check(self.L.xdem_log_count(self.h, C.byref(n)))
+        return n.value
+
+    def log_fetch(self, first: int = 0, rows: int | None = None):
+        if rows is None:
+            rows = self.log_count() - first
+        f = np.zeros((rows, MLP_IN), np.float32)
+        t = np.zeros(rows, np.float32)
+        it = np.zeros(rows, np.int32)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.L.xdem_log_fetch(self.h, int(first), int(rows), ptr(f), ptr(t), ptr(it)))
+        return f, t, it
+
+    def mlp_train(self, epochs: int, batch: int = 65536, lr: float = 1e-3, seed: int = 0,
+                  holdout: float = 0.1):
+        tr = np.zeros(epochs, np.float32)
+        ho = np.zeros(epochs, np.float32)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.L.xdem_mlp_train(self.h, int(epochs), int(batch), float(lr), int(seed),
+                                          float(holdout), ptr(tr), ptr(ho)))
+        return tr, ho
+
+    def mlp_grad(self, features, targets):
+        f = np.ascontiguousarray(features, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float(0)
+        self._check(self.L.xdem_mlp_grad(self.h, int(t.size), f, t, C.byref(loss), g))
+        return loss.value, g
+
+    def mlp_predict(self, features):
+        f = np.ascontiguousarray(features, np.float32)
+        out = np.zeros(f.shape[0], np.float32)
+        self._check(self.L.xdem_mlp_predict(self.h, int(f.shape[0]), f, out))
+        return out

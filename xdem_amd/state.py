@@ -58,6 +58,7 @@ class WorldConfig:
     newton_cap: int = 0          # 0: 200 updates
     strict_newton: int = 0       # 1: error instead of the bisection rescue
     skin: float = 0.0            # pair-list skin; 0: 0.2 * max outer radius
+    pair_blocks: int = 0         # workgroups of the pair kernels; 0: 8 per CU
 
     def cfg_array(self) -> np.ndarray:
         return np.array([getattr(self, k) for k in _CFG_KEYS], dtype=np.float64)
@@ -283,4 +284,52 @@ def add_clumps(s: State, num_clumps: int, seed: int = 0, sub_r: float = 0.35,
             s.clump_bx[p], s.clump_by[p] = s.x[p] - cx, s.y[p] - cy
             s.mass[p], s.inertia[p] = mc, ic
             s.vx[p] = s.vy[p] = s.omega[p] = 0.0
+    return s
+
+
+def make_ensemble(num: int = 256, seed: int = 0, n_range=(2000, 50000), phi: float = 0.8,
+                  rate_range=(1e-5, 1e-3), clump_share=(0.05, 0.25)) -> list:
+    """Seeded batch of independent packings of uneven size.
+
+    Each packing draws N uniformly from n_range and mixes, particle by
+    particle, circles (stored as aspect-ratio-1 ellipses: the reference
+    cannot pair a CircularParticle with an EllipticParticle,
+    src/EllipticParticle.cpp:30) and ellipses of aspect ratio U[1, 2.5]; half
+    of the packings are Lees-Edwards shear cells (rate log-uniform in
+    rate_range), the other half periodic cells that also hold 3-disc clumps
+    (GluedParticle; the Lees-Edwards world does not honour the clump's
+    SKIP_CONTACT_DETECTION, src/LeesEdwardsBoundaryWorld.cpp:141-199).
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(num):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        le = bool(rng.random() < 0.5)
+        s = make_packing(n, phi=phi, a_range=(0.4, 0.6), ar_range=(1.0, 2.5),
+                         world_kind=WORLD_LEES_EDWARDS if le else WORLD_PERIODIC,
+                         shear_rate=float(10 ** rng.uniform(np.log10(rate_range[0]),
+                                                           np.log10(rate_range[1]))) if le else 0.0,
+                         seed=int(rng.integers(1 << 31)))
+        circles = rng.random(n) < rng.uniform(0.0, 0.5)
+        s.aspect[circles] = 1.0
+        s.mass, s.inertia, _ = particle_properties(s.kind, s.r, s.aspect)
+        if not le:
+            nc = int(n * rng.uniform(*clump_share) / 4)
+            s = add_clumps(s, nc, seed=int(rng.integers(1 << 31)), sub_r=0.3, spacing=0.35)
+        out.append(s)
+    return out
+
+
+def clump_branches(s: State) -> State:
+    """Recompute the clumps' branch vectors (sub-particle centre minus clump
+    centre, minimum image) from the current positions, e.g. after a
+    relaxation that moved the sub-particles with their clumps."""
+    s = s.copy()
+    lx, ly = s.cfg.xmax - s.cfg.xmin, s.cfg.ymax - s.cfg.ymin
+    idx = np.arange(s.n)
+    for c in np.nonzero(s.kind == KIND_CLUMP)[0]:
+        subs = np.nonzero((s.up_level == c) & (idx != c))[0]
+        bx, by = s.x[subs] - s.x[c], s.y[subs] - s.y[c]
+        s.clump_bx[subs] = bx - lx * np.round(bx / lx)
+        s.clump_by[subs] = by - ly * np.round(by / ly)
     return s

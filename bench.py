@@ -205,6 +205,21 @@ def cpu_baseline(s, steps: int, threads: int):
             "sample": f"{steps} steps of the same {s.n}-particle packing (after 1 warm-up step)"}
 
 
+def copy_probe(reps: int = 5, n: int = 64 << 20):
+    """A plain streaming kernel of known traffic (b = 2a over n doubles:
+    8n bytes read, 8n bytes written per launch).  Run under rocprofv3 --pmc,
+    its FETCH_SIZE / WRITE_SIZE readings give the factor that turns those
+    counters into bytes for the engine's kernels."""
+    import torch
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    return {"kernel": "elementwise b = 2a", "doubles": n, "bytes_read": 8 * n,
+            "bytes_written": 8 * n, "launches": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +238,8 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--ens-relax", type=int, default=600)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a streaming kernel of known traffic (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU; multi-GPU runs "
@@ -306,6 +323,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline(s, args.cpu_steps, threads)
         except Exception as e:  # reported, never silently replaced
             out["cpu_baseline"] = {"error": str(e)}
+    if args.copy_probe:
+        out["copy_probe"] = copy_probe()
     print(json.dumps(out))
 
 

@@ -1534,6 +1534,9 @@ int xdem_create(xdem_engine** out, int num_packings, const xdem_config* cfgs) {
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, E->device);
     E->pair_grid = cfgs[0].pair_blocks > 0 ? cfgs[0].pair_blocks : cus * 8;
     E->prec = cfgs[0].precision;
+    // XDEM_GRAPHS=0: launch every step eagerly (for tools that cannot follow
+    // graph launches, e.g. per-dispatch counter collection)
+    if (const char* g = getenv("XDEM_GRAPHS")) E->use_graphs = strcmp(g, "0") != 0;
     {
         // network state (used by the learned-guess mode and the training calls)
         int rc;

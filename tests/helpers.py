@@ -80,3 +80,73 @@ def compare_step(a, b, scale_f):
     """max abs differences between two read-backs (dicts of arrays)."""
     return {k: float(np.max(np.abs(a[k] - b[k]))) if len(a[k]) else 0.0
             for k in a if k in b}
+
+
+def pw_f(dx, dy, a1, b1, th1, a2, b2, th2, lam):
+    """Perram-Wertheim contact function F(lam) of two ellipses (semi-axes a,
+    b, orientation th) whose centres differ by (dx, dy): the ellipses overlap
+    iff max over lam in [0, 1] of F is <= 1 (J. Comput. Phys. 58, 409)."""
+    c1, s1, c2, s2 = np.cos(th1), np.sin(th1), np.cos(th2), np.sin(th2)
+    g1, g2 = a1 * a1 - b1 * b1, a2 * a2 - b2 * b2
+    sxx = (1 - lam) * (b1 * b1 + g1 * c1 * c1) + lam * (b2 * b2 + g2 * c2 * c2)
+    sxy = (1 - lam) * (g1 * c1 * s1) + lam * (g2 * c2 * s2)
+    syy = (1 - lam) * (b1 * b1 + g1 * s1 * s1) + lam * (b2 * b2 + g2 * s2 * s2)
+    det = sxx * syy - sxy * sxy
+    q = syy * dx * dx - 2 * sxy * dx * dy + sxx * dy * dy
+    return lam * (1 - lam) * q / det
+
+
+def pw_touching_distance(ux, uy, a1, b1, th1, a2, b2, th2):
+    """Centre distance along the unit vector (ux, uy) at which the two
+    ellipses touch: F scales with the squared distance, so it is
+    1 / sqrt(max_lam F(lam; u)) (golden-section search, F is concave in lam)."""
+    lo, hi = np.zeros_like(ux), np.ones_like(ux)
+    gr = 0.5 * (np.sqrt(5.0) - 1.0)
+    for _ in range(90):
+        l1, l2 = hi - gr * (hi - lo), lo + gr * (hi - lo)
+        up = pw_f(ux, uy, a1, b1, th1, a2, b2, th2, l1) < pw_f(ux, uy, a1, b1, th1, a2, b2, th2, l2)
+        lo, hi = np.where(up, l1, lo), np.where(up, hi, l2)
+    return 1.0 / np.sqrt(pw_f(ux, uy, a1, b1, th1, a2, b2, th2, 0.5 * (lo + hi)))
+
+
+def near_touching_poses(n, seed, a_range=(0.4, 0.6), ar_range=(1.0, 2.5)):
+    """Random ellipse pairs at centre distance d0 (1 + eps), d0 the touching
+    distance, eps = +-10^-U(1, 9): half just apart, half just overlapping.
+    Returns dict of arrays (a1, ar1, th1, a2, ar2, th2, dx, dy, apart)."""
+    rng = np.random.default_rng(seed)
+    a1, a2 = rng.uniform(*a_range, n), rng.uniform(*a_range, n)
+    ar1, ar2 = rng.uniform(*ar_range, n), rng.uniform(*ar_range, n)
+    th1, th2 = rng.uniform(0, np.pi, n), rng.uniform(0, np.pi, n)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    ux, uy = np.cos(ang), np.sin(ang)
+    d0 = pw_touching_distance(ux, uy, a1, a1 / ar1, th1, a2, a2 / ar2, th2)
+    apart = rng.uniform(size=n) < 0.5
+    eps = 10.0 ** -rng.uniform(1, 9, n)
+    d = d0 * np.where(apart, 1 + eps, 1 - eps)
+    return dict(a1=a1, ar1=ar1, th1=th1, a2=a2, ar2=ar2, th2=th2, dx=d * ux, dy=d * uy,
+                apart=apart)
+
+
+def pair_lattice_state(poses, spacing=4.0, world_kind=WORLD_PERIODIC, shear_rate=0.0):
+    """A packing of isolated ellipse pairs (one pose per lattice site, far
+    enough apart that only the two ellipses of a site can touch)."""
+    from xdem_amd.state import KIND_ELLIPSE, State, WorldConfig, particle_properties
+    n = len(poses["a1"])
+    side = int(np.ceil(np.sqrt(n)))
+    k = np.arange(n)
+    cx, cy = (k % side + 0.5) * spacing, (k // side + 0.5) * spacing
+    x = np.empty(2 * n); y = np.empty(2 * n)
+    x[0::2], y[0::2] = cx - 0.5 * poses["dx"], cy - 0.5 * poses["dy"]
+    x[1::2], y[1::2] = cx + 0.5 * poses["dx"], cy + 0.5 * poses["dy"]
+    r = np.empty(2 * n); ar = np.empty(2 * n); th = np.empty(2 * n)
+    r[0::2], r[1::2] = poses["a1"], poses["a2"]
+    ar[0::2], ar[1::2] = poses["ar1"], poses["ar2"]
+    th[0::2], th[1::2] = poses["th1"], poses["th2"]
+    kind = np.full(2 * n, KIND_ELLIPSE, np.int32)
+    m, inertia, _ = particle_properties(kind, r, ar)
+    L = side * spacing
+    cfg = WorldConfig(world_kind=world_kind, xmin=0.0, xmax=L, ymin=0.0, ymax=L,
+                      dt=1e-4, shear_rate=shear_rate)
+    z = np.zeros(2 * n)
+    return State(cfg, kind, np.arange(2 * n, dtype=np.int32), np.zeros(2 * n, np.int32),
+                 r, ar, m, inertia, x, y, th, z.copy(), z.copy(), z.copy(), z.copy(), z.copy())

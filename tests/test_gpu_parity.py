@@ -305,3 +305,25 @@ def test_nonfinite_geometry_repair_parity():
     gpu = run_gpu(s)
     assert gpu["contacts"]["i"].size == 1 and np.all(np.isfinite(gpu["forces"]["fx"]))
     assert_parity(gpu, ref, env, "repaired pair")
+
+
+@pytest.mark.parametrize("world", ["periodic", "lees_edwards"])
+def test_separation_pretest_keeps_contact_decisions(world):
+    """Isolated ellipse pairs within 10^-1..10^-9 (relative) of touching, half
+    apart and half overlapping: the engine, which spares the contact kernel
+    the pairs its Perram-Wertheim pre-test proves apart, reports exactly the
+    oracle's contact set (the reference's decisions, quirks included)."""
+    from xdem_amd.state import WORLD_LEES_EDWARDS, WORLD_PERIODIC
+    poses = helpers.near_touching_poses(3000, seed=21 if world == "periodic" else 22)
+    kind = WORLD_PERIODIC if world == "periodic" else WORLD_LEES_EDWARDS
+    s = helpers.pair_lattice_state(poses, world_kind=kind,
+                                   shear_rate=0.0 if world == "periodic" else 0.3)
+    w = po.OracleWorld(s)
+    w.step(1)
+    co = w.contacts()
+    eng = _engine(s)
+    eng.step(1)
+    ce = eng.contacts()
+    print(f"[gpu] pre-test {world}: {ce['i'].size} contacts of {len(poses['a1'])} pairs", flush=True)
+    assert ce["i"].size > 0.3 * len(poses["a1"])
+    assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"])

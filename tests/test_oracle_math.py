@@ -336,3 +336,42 @@ def test_reference_returns_nan_for_that_pose(tmp_path):
     row = np.array([[1.0] + NAN_POSE + [0.0]])
     g = po.reference_pairs(row, str(tmp_path))[0]
     assert g[0] == 1.0 and np.isnan(g[1])
+
+
+def test_pw_half_separation_implies_no_oracle_contact():
+    """The contact kernel is spared every ellipse pair whose Perram-Wertheim
+    contact function at lambda = 1/2 exceeds 1 + 1e-6 (such a pair is
+    provably apart).  Check against the oracle's contact decision, which is
+    the reference's arithmetic including its quirks: no culled pose may be an
+    oracle contact, and no overlapping pose may be culled.  Poses: random
+    pairs within 10^-1..10^-9 (relative) of touching, and pairs uniform in the
+    bounding-circle disc."""
+    from tests import helpers
+    p = helpers.near_touching_poses(6000, seed=11)
+    rng = np.random.default_rng(12)
+    n = 6000
+    u = dict(a1=rng.uniform(0.4, 0.6, n), ar1=rng.uniform(1, 2.5, n), th1=rng.uniform(0, np.pi, n),
+             a2=rng.uniform(0.4, 0.6, n), ar2=rng.uniform(1, 2.5, n), th2=rng.uniform(0, np.pi, n))
+    ang = rng.uniform(0, 2 * np.pi, n)
+    rad = (u["a1"] + u["a2"]) * np.sqrt(rng.uniform(0, 1, n))
+    u["dx"], u["dy"] = rad * np.cos(ang), rad * np.sin(ang)
+    culled_total = 0
+    noncontact_uniform = culled_uniform = 0
+    for name, q in (("near", p), ("uniform", u)):
+        f = helpers.pw_f(q["dx"], q["dy"], q["a1"], q["a1"] / q["ar1"], q["th1"],
+                         q["a2"], q["a2"] / q["ar2"], q["th2"], 0.5)
+        cull = f > 1 + 1e-6
+        if name == "near":
+            assert not np.any(cull & ~q["apart"]), "an overlapping pose was culled"
+        for k in range(len(f)):
+            o = po.pair_geometry(1, q["a1"][k], q["ar1"][k], 0.0, 0.0, q["th1"][k],
+                                 q["a2"][k], q["ar2"][k], q["dx"][k], q["dy"][k], q["th2"][k])
+            if cull[k]:
+                assert o[0] == 0, f"{name} pose {k}: culled but the oracle reports a contact"
+            if name == "uniform" and o[0] == 0:
+                noncontact_uniform += 1
+                culled_uniform += int(cull[k])
+        culled_total += int(cull.sum())
+    assert culled_total > 1000
+    # the test is worth running: most pairs apart are caught
+    assert culled_uniform > 0.7 * noncontact_uniform

@@ -108,28 +108,42 @@ struct Parts {
     int n;
 };
 
-// Pair / contact table.  Rows hold a particle's listed partners j > i in
-// ascending ID order; ct_j < 0 marks a listed pair without a contact.
+// Pair list and contacts of one step.
+// Slots: rows hold a particle's listed partners j > i in ascending ID order
+// (pr_i, pr_j; the list is rewritten only at rebuilds and shared by both
+// step parities); link[slot] is the index of the pair's contact in the
+// contact list of this step, or -1.
+// Contact list: compact arrays written in work-item order, a wavefront's
+// contacts at consecutive indices, so every write of the contact kernel is
+// a full, coalesced line (slot-ordered writes leave holes where listed pairs
+// are not in contact, and partial-line writes were the kernel's main cost).
+// Consumers always go through the slots, so the sums keep the reference's
+// order.
 struct Table {
-    int *pr_i, *pr_j, *ct_j;
-    int* ej;                      // external index of the partner (ordering key)
+    int *pr_i, *pr_j;
+    int* link;
+    int *row_start, *row_cnt;
+    int* counter;                 // kSeg row-allocation counters, kCtrStride apart
+    int cap;                      // slots
+    int seg_cap;                  // slots per segment
+    // contact list
+    int *ci, *cj;                 // P1, P2 (internal indices)
     double *xix, *xiy, *nx, *ny, *c;
     FRec* fr;
     double *fnx, *fny, *fsx, *fsy, *bx, *by, *px, *py;
-    int *row_start, *row_cnt;
-    int* counter;                 // kSeg counters, kCtrStride apart
-    int cap;                      // total entries
-    int seg_cap;                  // entries per segment
+    int* ccounter;                // kSeg contact-allocation counters
+    int cseg_cap;                 // contacts per segment
 };
 
 // Pairs that passed the bounding-circle test this step: x = slot | (image
 // list << kLstShift), y = slot of the pair's history in the previous table
-// or -1.  Segmented like the table.
+// or -1, z = P1, w = P2 (so the contact kernel starts its particle loads
+// without first reading the pair list).  Segmented like the table.
 constexpr int kLstShift = 30;
 // In the learned-guess mode each item also carries the pair's features
 // (BF16) and the network's guess.
 struct WorkList {
-    int2* item;
+    int4* item;
     uint4* feat;
     float* guess;
     int* counter;                 // kSeg counters, kCtrStride apart
@@ -185,6 +199,82 @@ __device__ __forceinline__ Counters* counter_slot(Counters* c) {
     return c + (blockIdx.x % kSeg);
 }
 
+// XCD-aware workgroup numbering.  gfx950 dispatches workgroup b to XCD
+// b % 8, and each XCD has its own L2.  The logical number returned here
+// gives XCD x the contiguous range of logical workgroups
+// [x*G/8, (x+1)*G/8), so the particles and pairs one XCD works on are
+// neighbours in memory (Morton order) and share its L2 instead of being
+// fetched into all eight.  A bijection of [0, G) for any grid size G.
+constexpr int kXcd = 8;
+
+__device__ __forceinline__ int xcd_block() {
+    const int b = (int)blockIdx.x, g = (int)gridDim.x;
+    const int per = g / kXcd, rem = g % kXcd;
+    const int x = b % kXcd, k = b / kXcd;
+    return x * per + min(x, rem) + k;
+}
+
+// Contiguous range [beg, end) of `total` work items for this workgroup: the
+// items are split into gridDim.x chunks (multiples of the block size) in
+// logical-workgroup order.
+__device__ __forceinline__ void xcd_chunk(int total, int& beg, int& end) {
+    const int g = (int)gridDim.x;
+    const int per = (total + g - 1) / g;
+    const int chunk = (per + kBlock - 1) / kBlock * kBlock;
+    const long long b = (long long)xcd_block() * chunk;
+    beg = (int)min(b, (long long)total);
+    end = (int)min(b + chunk, (long long)total);
+}
+
+// Segment of a table / work list that this workgroup appends to: logical
+// workgroups map to segments in order, so segment order follows particle
+// order and each segment is filled by workgroups of one XCD.
+__device__ __forceinline__ int xcd_segment() {
+    return (int)((long long)xcd_block() * kSeg / (long long)gridDim.x);
+}
+
+// Contact-list index for one contact (rare paths: one atomic each); -1 and
+// an error bit when the segment is full.
+__device__ __forceinline__ int alloc_contact_one(const Table& t, uint32_t* err) {
+    const int seg = xcd_segment();
+    const int k = atomicAdd(t.ccounter + seg * kCtrStride, 1);
+    if (k >= t.cseg_cap) { raise_err(err, ERR_PAIR_OVERFLOW); return -1; }
+    return seg * t.cseg_cap + k;
+}
+
+// Contact-list indices for the lanes of a wavefront with `want` set:
+// consecutive indices in lane order, one atomic per wavefront.  Every
+// active lane of the wavefront must call it (converged control flow).
+__device__ __forceinline__ int alloc_contact_wave(const Table& t, bool want, uint32_t* err) {
+    const unsigned long long m = __ballot(want ? 1 : 0);
+    if (m == 0ULL) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int cnt = __popcll(m);
+    const int seg = xcd_segment();
+    int base = 0;
+    if (lane == leader) base = atomicAdd(t.ccounter + seg * kCtrStride, cnt);
+    base = __shfl(base, leader, 64);
+    if (base + cnt > t.cseg_cap) {
+        if (lane == leader) raise_err(err, ERR_PAIR_OVERFLOW);
+        return -1;
+    }
+    if (!want) return -1;
+    return seg * t.cseg_cap + base + __popcll(m & ((1ULL << lane) - 1ULL));
+}
+
+// Copy one contact record between contact lists.
+__device__ __forceinline__ void copy_contact(const Table& from, int a, const Table& to, int b) {
+    to.ci[b] = from.ci[a]; to.cj[b] = from.cj[a];
+    to.xix[b] = from.xix[a]; to.xiy[b] = from.xiy[a];
+    to.nx[b] = from.nx[a]; to.ny[b] = from.ny[a]; to.c[b] = from.c[a];
+    to.fr[b] = from.fr[a];
+    to.fnx[b] = from.fnx[a]; to.fny[b] = from.fny[a];
+    to.fsx[b] = from.fsx[a]; to.fsy[b] = from.fsy[a];
+    to.bx[b] = from.bx[a]; to.by[b] = from.by[a];
+    to.px[b] = from.px[a]; to.py[b] = from.py[a];
+}
+
 // Image offset of P2 for a pair found in bin row `img`:
 // 0: minimum image (src/PeriodicBoundaryWorld.cpp:97-101,
 //    src/LeesEdwardsBoundaryWorld.cpp:218-222);
@@ -220,9 +310,12 @@ template <int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
       WorkList W, const int* __restrict__ rebuild, int* rebuild_next, uint32_t* err) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int p = xcd_block() * kBlock + threadIdx.x;
     if (p == 0) *rebuild_next = 0;      // set again by k_integrate when needed
-    if (p < kSeg) W.counter[p * kCtrStride] = 0;
+    if (p < kSeg) {
+        W.counter[p * kCtrStride] = 0;
+        new_t.ccounter[p * kCtrStride] = 0;
+    }
     if (p >= P.n || *err) return;
     const bool rb = *rebuild != 0;
     if (!rb && p < kSeg) new_t.counter[p * kCtrStride] = old_t.counter[p * kCtrStride];
@@ -378,7 +471,7 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     if (*rebuild == 0) return;
     const int tid = threadIdx.x;
     if (blockIdx.x == 0 && tid == 0) atomicAdd(&cnt->builds, 1ULL);
-    const int i = blockIdx.x * kBlock + tid;
+    const int i = xcd_block() * kBlock + tid;
     const bool active = i < P.n && *err == 0;
     int nl = 0;
     unsigned long long ncand_total = 0;
@@ -435,8 +528,9 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
         const int nlisted = min(nl, kMaxList);
         const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
         for (int e = os; e < os + oc; ++e) {
-            const int j = old_t.ct_j[e];
-            if (j < 0) continue;
+            const int c = old_t.link[e];
+            if (c < 0) continue;
+            const int j = old_t.cj[c];
             bool listed = false;
             for (int k = 0; k < nlisted; ++k) listed |= s_list[k][tid] == j;
             if (listed) continue;
@@ -466,7 +560,7 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     }
     if (tid == kBlock - 1) {
         const int block_total = s_scan[tid];
-        const int seg = blockIdx.x % kSeg;
+        const int seg = xcd_segment();
         int base = block_total ? atomicAdd(new_t.counter + seg * kCtrStride, block_total) : 0;
         if ((long long)base + block_total > new_t.seg_cap) {
             raise_err(err, ERR_PAIR_OVERFLOW);
@@ -498,7 +592,7 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
 // Rebuild steps: order each particle's incoming index by the ID of P1.
 __global__ void __launch_bounds__(kBlock)
 k_in_sort(Parts P, const int* __restrict__ rebuild, uint32_t* err) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int p = xcd_block() * kBlock + threadIdx.x;
     if (*rebuild == 0 || p >= P.n || *err) return;
     const size_t n = (size_t)P.n;
     int2* refs = P.in_ref + p;                   // entry k at refs[k * n]
@@ -548,6 +642,35 @@ __device__ __forceinline__ int segment_of(const int* s_pre, int w) {
 // item also receives the pair's features.
 constexpr float kCullMargin = 1.000001f;
 
+// Separation test of two ellipses from the Perram-Wertheim contact function
+// (J. W. Perram & M. S. Wertheim, J. Comput. Phys. 58, 409 (1985)):
+// F(lam) = lam (1 - lam) d^T [(1 - lam) A1^-1 + lam A2^-1]^-1 d, with
+// A^-1 = b^2 I + (a^2 - b^2) k k^T; the ellipses overlap iff max F <= 1.
+// Evaluated at lam = 1/2, F = d^T adj(S) d / (2 det S), S = A1^-1 + A2^-1,
+// so F > 1 + margin proves the pair separated by a relative distance of
+// about margin / 2, and the pair cannot be a contact: the contact test
+// (src/EllipticParticle.cpp:178-186) only reports overlaps.  Pairs that pass
+// are decided by the reference's own arithmetic; this test only spares the
+// ZPM computation for pairs that are clearly apart.  Not used in strict
+// mode, which reports every pair the reference's Newton loop would hang on.
+constexpr double kPwMargin = 1e-6;
+constexpr float kPwMarginF32 = 1e-3f;
+
+template <class T>
+__device__ __forceinline__ bool pw_separated(T dx, T dy, double a1, double b1, double k1x,
+                                             double k1y, double a2, double b2, double k2x,
+                                             double k2y, T margin) {
+    const T g1 = (T)(a1 * a1 - b1 * b1), g2 = (T)(a2 * a2 - b2 * b2);
+    const T c1x = (T)k1x, c1y = (T)k1y, c2x = (T)k2x, c2y = (T)k2y;
+    const T bb = (T)(b1 * b1 + b2 * b2);
+    const T sxx = bb + g1 * c1x * c1x + g2 * c2x * c2x;
+    const T sxy = g1 * c1x * c1y + g2 * c2x * c2y;
+    const T syy = bb + g1 * c1y * c1y + g2 * c2y * c2y;
+    const T det = sxx * syy - sxy * sxy;
+    const T q = syy * dx * dx - (T)2 * sxy * dx * dy + sxx * dy * dy;
+    return q > (T)2 * ((T)1 + margin) * det;
+}
+
 template <int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t, WorkList W,
@@ -557,41 +680,45 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
     __shared__ int s_base;
     __shared__ int s_go;
     const bool rb = *rebuild != 0;
-    const int* __restrict__ src_i = rb ? new_t.pr_i : old_t.pr_i;
-    const int* __restrict__ src_j = rb ? new_t.pr_j : old_t.pr_j;
+    // pr_i / pr_j are shared by both tables (rewritten only at rebuilds)
+    const int* __restrict__ src_i = new_t.pr_i;
+    const int* __restrict__ src_j = new_t.pr_j;
     if (threadIdx.x == 0) {
         s_go = *err == 0;
         segment_prefix(s_pre, rb ? new_t.counter : old_t.counter, new_t.seg_cap);
     }
     __syncthreads();
     if (!s_go) return;
-    const int total = s_pre[kSeg];
+    int wbeg, wend;
+    xcd_chunk(s_pre[kSeg], wbeg, wend);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long n_coarse = 0;
-    for (int w0 = blockIdx.x * kBlock; w0 < total; w0 += gridDim.x * kBlock) {
+    for (int w0 = wbeg; w0 < wend; w0 += kBlock) {
         const int w = w0 + threadIdx.x;
         bool push = false;
-        int2 item = make_int2(0, -1);
+        int4 item = make_int4(0, -1, 0, 0);
         float feat[kMlpIn];
         if (PREC == kNN)
             for (int k = 0; k < kMlpIn; ++k) feat[k] = 0.f;
-        if (w < total) {
+        if (w < wend) {
             const int lo = segment_of(s_pre, w);
             const int e = lo * new_t.seg_cap + (w - s_pre[lo]);
             const int i = src_i[e];
             const int code = src_j[e];
             const int j = code & ~kExtraBit;
-            if (!rb) { new_t.pr_i[e] = i; new_t.pr_j[e] = code; }
             if (i >= 0 && i < P.n && j >= 0 && j < P.n) {
                 const Pack K = packs[P.pack[i]];
-                // history of (i, j): the same slot when the list was not rebuilt
+                // history of (i, j) as an index into the previous contact
+                // list: the same slot when the list was not rebuilt
                 int found = -1;
                 if (!rb) {
-                    if (old_t.ct_j[e] == j) found = e;
+                    found = old_t.link[e];
                 } else {
                     const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
-                    for (int t = os; t < os + oc; ++t)
-                        if (old_t.ct_j[t] == j) { found = t; break; }
+                    for (int t = os; t < os + oc; ++t) {
+                        const int c = old_t.link[t];
+                        if (c >= 0 && old_t.cj[c] == j) { found = c; break; }
+                    }
                 }
                 const int ki = P.kind[i];
                 int lst;
@@ -611,6 +738,10 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
                         const float rs = (float)(P.r[i] + P.r[j]) * kCullMargin;
                         push = rs >= sqrtf(dxf * dxf + dyf * dyf);
                         n_coarse += (ki == KIND_CIRCLE || push) ? 1 : 0;
+                        if (push && ki == KIND_ELLIPSE && !K.strict)
+                            push = !pw_separated<float>(dxf, dyf, P.r[i], F.b[i], F.kx[i], F.ky[i],
+                                                        P.r[j], F.b[j], F.kx[j], F.ky[j],
+                                                        kPwMarginF32);
                     } else if (ki == KIND_CIRCLE) {
                         const double bvx = (xj + ox) - xi, bvy = (yj + oy) - yi;
                         push = !((P.r[i] + P.r[j]) - sqrt(bvx * bvx + bvy * bvy) < 0.0);
@@ -619,6 +750,10 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
                         const double qx = xi - (xj + ox), qy = yi - (yj + oy);
                         push = (P.r[i] + P.r[j]) > sqrt(qx * qx + qy * qy);
                         n_coarse += push ? 1 : 0;
+                        if (push && !K.strict)
+                            push = !pw_separated<double>(-qx, -qy, P.r[i], F.b[i], F.kx[i], F.ky[i],
+                                                         P.r[j], F.b[j], F.kx[j], F.ky[j],
+                                                         kPwMargin);
                         if (PREC == kNN && push)
                             root_features((float)((xj + ox) - xi), (float)((yj + oy) - yi),
                                           (float)F.kx[i], (float)F.ky[i], (float)F.kx[j],
@@ -627,23 +762,16 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
                     }
                 }
                 if (push) {
-                    item = make_int2((int)((unsigned)e | ((unsigned)lst << kLstShift)), found);
+                    item = make_int4((int)((unsigned)e | ((unsigned)lst << kLstShift)), found, i, j);
                 } else if (found >= 0 && lst < 0) {     // not a candidate: carried forward
-                    new_t.ct_j[e] = j;
-                    new_t.ej[e] = old_t.ej[found];
-                    new_t.xix[e] = old_t.xix[found]; new_t.xiy[e] = old_t.xiy[found];
-                    new_t.nx[e] = old_t.nx[found];   new_t.ny[e] = old_t.ny[found];
-                    new_t.c[e] = old_t.c[found];
-                    new_t.fr[e] = old_t.fr[found];
-                    new_t.fnx[e] = old_t.fnx[found]; new_t.fny[e] = old_t.fny[found];
-                    new_t.fsx[e] = old_t.fsx[found]; new_t.fsy[e] = old_t.fsy[found];
-                    new_t.bx[e] = old_t.bx[found];   new_t.by[e] = old_t.by[found];
-                    new_t.px[e] = old_t.px[found];   new_t.py[e] = old_t.py[found];
+                    const int c = alloc_contact_one(new_t, err);
+                    if (c >= 0) copy_contact(old_t, found, new_t, c);
+                    new_t.link[e] = c;
                 } else {
-                    new_t.ct_j[e] = -1;
+                    new_t.link[e] = -1;
                 }
             } else {
-                new_t.ct_j[e] = -1;
+                new_t.link[e] = -1;
             }
         }
         // block compaction: one counter update per block and round
@@ -660,7 +788,7 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
             }
             int base = -1;
             if (tot > 0) {
-                const int seg = blockIdx.x % kSeg;
+                const int seg = xcd_segment();
                 base = atomicAdd(W.counter + seg * kCtrStride, tot);
                 if ((long long)base + tot > W.seg_cap) {
                     raise_err(err, ERR_PAIR_OVERFLOW);
@@ -777,6 +905,11 @@ struct StepCounts {
 // CircularParticle::contactWithCircularParticle, src/CircularParticle.cpp:67-92),
 // history and contact model (BaseParticle::calculateContactForceAndTorque,
 // src/BaseParticle.cpp:27-98).  `guess`: the network's output (learned mode).
+// `e`: the pair's slot; `found`: index of the pair's history in the
+// previous contact list, or -1.  A contact gets the next free entry of the
+// contact list (wavefront-compacted, so the writes below are contiguous),
+// and link[e] points to it.  Every active lane of the wavefront must call
+// this function (it allocates per wavefront once the geometry is known).
 template <int PREC>
 __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, const Pack& K,
                                              const Table& old_t, const Table& new_t, int e,
@@ -826,10 +959,9 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
         if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); hit = 0; }
         if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); hit = 0; }
     }
-    if (!hit) {
-        new_t.ct_j[e] = -1;
-        return;
-    }
+    const int ct = alloc_contact_wave(new_t, hit != 0, err);
+    new_t.link[e] = ct;
+    if (ct < 0) return;
     const bool le = K.world == WORLD_LE;
     double xix, xiy, c, fnx, fny, fsx, fsy, t1, t2, fx, fy;
     if (PREC == kMixed) {
@@ -954,19 +1086,18 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
         t1 = g.b1x * fy - g.b1y * fx;
         t2 = g.b2x * (-fy) - g.b2y * (-fx);
     }
-    new_t.ct_j[e] = j;
-    new_t.ej[e] = P.eid[j];
-    new_t.xix[e] = xix; new_t.xiy[e] = xiy;
-    new_t.nx[e] = g.nx; new_t.ny[e] = g.ny;
-    new_t.c[e] = c;
+    new_t.ci[ct] = i; new_t.cj[ct] = j;
+    new_t.xix[ct] = xix; new_t.xiy[ct] = xiy;
+    new_t.nx[ct] = g.nx; new_t.ny[ct] = g.ny;
+    new_t.c[ct] = c;
     FRec r;
     r.fx = fx; r.fy = fy; r.t1 = t1; r.t2 = t2;
-    new_t.fr[e] = r;
-    new_t.fnx[e] = fnx; new_t.fny[e] = fny;
-    new_t.fsx[e] = fsx; new_t.fsy[e] = fsy;
-    new_t.bx[e] = xi - xj2;
-    new_t.by[e] = yi - yj2;
-    new_t.px[e] = g.px; new_t.py[e] = g.py;
+    new_t.fr[ct] = r;
+    new_t.fnx[ct] = fnx; new_t.fny[ct] = fny;
+    new_t.fsx[ct] = fsx; new_t.fsy[ct] = fsy;
+    new_t.bx[ct] = xi - xj2;
+    new_t.by[ct] = yi - yj2;
+    new_t.px[ct] = g.px; new_t.py[ct] = g.py;
 }
 
 // One lane per work item.
@@ -982,19 +1113,19 @@ k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table n
     }
     __syncthreads();
     if (!s_go) return;
-    const int total = s_pre[kSeg];
+    int wbeg, wend;
+    xcd_chunk(s_pre[kSeg], wbeg, wend);
     StepCounts n;
-    for (int w = blockIdx.x * kBlock + threadIdx.x; w < total; w += gridDim.x * kBlock) {
+    for (int w = wbeg + threadIdx.x; w < wend; w += kBlock) {
         const int lo = segment_of(s_pre, w);
         const int at = lo * W.seg_cap + (w - s_pre[lo]);
-        const int2 it = W.item[at];
+        const int4 it = W.item[at];
         const unsigned ux = (unsigned)it.x;
         const int e = (int)(ux & ((1u << kLstShift) - 1u));
         const int lst = (int)(ux >> kLstShift);
-        const int i = new_t.pr_i[e];
-        const int j = new_t.pr_j[e] & ~kExtraBit;
-        const Pack K = packs[P.pack[i]];
+        const int i = it.z, j = it.w;
         const float guess = PREC == kNN ? W.guess[at] : 0.0f;
+        const Pack K = packs[P.pack[i]];
         contact_item<PREC>(P, F, K, old_t, new_t, e, lst, it.y, i, j, guess, err, n);
     }
     Counters* cs = counter_slot(cnt);
@@ -1019,9 +1150,13 @@ k_guess(WorkList W, const MlpPacked* __restrict__ pk, const uint32_t* err) {
     const int l = threadIdx.x & 63, g = l >> 4, r = l & 15;
     MlpRegs R;
     mlp_load(pk, R);
+    // 16 items per tile, one tile per wavefront and round; contiguous tile
+    // ranges per workgroup in XCD order
+    constexpr int wpb = kBlock / 64;
     const int ntiles = (total + 15) / 16;
-    const int nw = gridDim.x * (kBlock / 64);
-    for (int tile = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nw) {
+    const int per = ((ntiles + (int)gridDim.x - 1) / (int)gridDim.x + wpb - 1) / wpb * wpb;
+    const int tbeg = min(xcd_block() * per, ntiles), tend = min(tbeg + per, ntiles);
+    for (int tile = tbeg + (threadIdx.x >> 6); tile < tend; tile += wpb) {
         const int w = tile * 16 + r;
         int at = -1;
         if (w < total) {
@@ -1052,22 +1187,21 @@ k_log(Parts P, Frame F, const Pack* __restrict__ packs, Table new_t, WorkList W,
     }
     __syncthreads();
     if (!s_go) return;
-    const int total = s_pre[kSeg];
+    int wbeg, wend;
+    xcd_chunk(s_pre[kSeg], wbeg, wend);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int w0 = blockIdx.x * kBlock; w0 < total; w0 += gridDim.x * kBlock) {
+    for (int w0 = wbeg; w0 < wend; w0 += kBlock) {
         const int w = w0 + threadIdx.x;
         bool keep = false;
         float feat[kMlpIn];
         float u = 0.f;
         int its = 0;
-        if (w < total) {
+        if (w < wend) {
             const int lo = segment_of(s_pre, w);
-            const int2 it = W.item[lo * W.seg_cap + (w - s_pre[lo])];
+            const int4 it = W.item[lo * W.seg_cap + (w - s_pre[lo])];
             const unsigned ux = (unsigned)it.x;
-            const int e = (int)(ux & ((1u << kLstShift) - 1u));
             const int lst = (int)(ux >> kLstShift);
-            const int i = new_t.pr_i[e];
-            const int j = new_t.pr_j[e] & ~kExtraBit;
+            const int i = it.z, j = it.w;
             if (P.kind[i] == KIND_ELLIPSE) {
                 const Pack K = packs[P.pack[i]];
                 const double xi = P.x[i], yi = P.y[i];
@@ -1134,16 +1268,16 @@ template <int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_gather(Parts P, Table t, uint32_t* err) {
     typedef typename std::conditional<PREC == kMixed, float, double>::type acc_t;
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int p = xcd_block() * kBlock + threadIdx.x;
     if (p >= P.n || *err) return;
     const size_t n = (size_t)P.n;
     const int nin = min(P.in_cnt[p], kInCap);
     acc_t fx = 0, fy = 0, tq = 0;
     int nc = 0;
     for (int k = 0; k < nin; ++k) {
-        const int e = P.in_ref[k * n + p].y;
-        if (t.ct_j[e] < 0) continue;
-        const FRec r = t.fr[e];
+        const int c = t.link[P.in_ref[k * n + p].y];
+        if (c < 0) continue;
+        const FRec r = t.fr[c];
         fx += -(acc_t)r.fx;
         fy += -(acc_t)r.fy;
         tq += (acc_t)r.t2;
@@ -1151,8 +1285,9 @@ k_gather(Parts P, Table t, uint32_t* err) {
     }
     const int s = t.row_start[p], cnt = t.row_cnt[p];
     for (int e = s; e < s + cnt; ++e) {
-        if (t.ct_j[e] < 0) continue;
-        const FRec r = t.fr[e];
+        const int c = t.link[e];
+        if (c < 0) continue;
+        const FRec r = t.fr[c];
         fx += (acc_t)r.fx;
         fy += (acc_t)r.fy;
         tq += (acc_t)r.t1;
@@ -1169,7 +1304,7 @@ k_gather(Parts P, Table t, uint32_t* err) {
 __global__ void __launch_bounds__(kBlock)
 k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
             int* rebuild_next, uint32_t* err) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int p = xcd_block() * kBlock + threadIdx.x;
     if (*rebuild)
         for (int b = p; b < B.nbins; b += gridDim.x * kBlock) B.count[b] = 0;
     if (p < kSeg) next_counter[p * kCtrStride] = 0;
@@ -1273,7 +1408,7 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
 // index order.
 __global__ void __launch_bounds__(kBlock)
 k_stress(Parts P, Table t, double* out) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int p = xcd_block() * kBlock + threadIdx.x;
     if (p >= P.n) return;
     double sxx = 0, sxy = 0, syx = 0, syy = 0;
     if (!(P.flags[p] & SKIP_MACRO_SUM)) {
@@ -1285,10 +1420,11 @@ k_stress(Parts P, Table t, double* out) {
             const double ox = clump ? P.cbx[s] : 0.0, oy = clump ? P.cby[s] : 0.0;
             const int rs = t.row_start[s], cnt = t.row_cnt[s];
             for (int e = rs; e < rs + cnt; ++e) {
-                if (t.ct_j[e] < 0) continue;
-                const double fx = t.fr[e].fx, fy = t.fr[e].fy;
-                const double bx = clump ? t.bx[e] + ox : t.bx[e];
-                const double by = clump ? t.by[e] + oy : t.by[e];
+                const int c = t.link[e];
+                if (c < 0) continue;
+                const double fx = t.fr[c].fx, fy = t.fr[c].fy;
+                const double bx = clump ? t.bx[c] + ox : t.bx[c];
+                const double by = clump ? t.by[c] + oy : t.by[c];
                 sxx += fx * bx; sxy += fx * by; syx += fy * bx; syy += fy * by;
             }
         }
@@ -1316,22 +1452,22 @@ __global__ void k_shape(int n, const double* r, const double* ar, const int8_t* 
     b[p] = vb; e2[p] = ve2; bsq[p] = vbsq; s1[p] = vs1; eta[p] = veta;
 }
 
-// history restart: one contact per thread into the old table's rows
+// history restart: one contact per thread; contact k of the restored list
+// sits in slot slot[k] of the old table's rows
 __global__ void k_load_contacts(Table t, int nc, const int* ci, const int* cj,
-                                const int* ej, const double* xix, const double* xiy,
-                                const double* nx, const double* ny, const double* c,
-                                const int* slot) {
+                                const double* xix, const double* xiy, const double* nx,
+                                const double* ny, const double* c, const int* slot) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nc) return;
     const int e = slot[k];
     t.pr_i[e] = ci[k];
     t.pr_j[e] = cj[k];
-    t.ct_j[e] = cj[k];
-    t.ej[e] = ej[k];
-    t.xix[e] = xix[k]; t.xiy[e] = xiy[k]; t.nx[e] = nx[k]; t.ny[e] = ny[k]; t.c[e] = c[k];
-    t.fnx[e] = 0; t.fny[e] = 0; t.fsx[e] = 0; t.fsy[e] = 0;
-    t.fr[e] = FRec{0.0, 0.0, 0.0, 0.0};
-    t.bx[e] = 0; t.by[e] = 0; t.px[e] = 0; t.py[e] = 0;
+    t.link[e] = k;
+    t.ci[k] = ci[k]; t.cj[k] = cj[k];
+    t.xix[k] = xix[k]; t.xiy[k] = xiy[k]; t.nx[k] = nx[k]; t.ny[k] = ny[k]; t.c[k] = c[k];
+    t.fnx[k] = 0; t.fny[k] = 0; t.fsx[k] = 0; t.fsy[k] = 0;
+    t.fr[k] = FRec{0.0, 0.0, 0.0, 0.0};
+    t.bx[k] = 0; t.by[k] = 0; t.px[k] = 0; t.py[k] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1460,19 +1596,33 @@ int check_error(xdem_engine* E) {
     return map_error_bits(E, bits);
 }
 
-int alloc_table(xdem_engine* E, Table& t, int cap, int n) {
+// `share`: a table whose pair list (pr_i, pr_j) this one uses too -- the
+// list is rewritten only at rebuilds and is the same for both step parities.
+// `cseg_cap`: contacts per segment of the contact list.
+int alloc_table(xdem_engine* E, Table& t, int cap, int n, const Table* share, int cseg_cap) {
     int rc;
     t.seg_cap = cap / kSeg;
     t.cap = t.seg_cap * kSeg;
     cap = t.cap;
-    if ((rc = dalloc(E, &t.pr_i, cap)) || (rc = dalloc(E, &t.pr_j, cap)) ||
-        (rc = dalloc(E, &t.ct_j, cap)) || (rc = dalloc(E, &t.ej, cap)))
+    t.cseg_cap = cseg_cap;
+    const size_t ccap = (size_t)cseg_cap * kSeg;
+    if (share) {
+        t.pr_i = share->pr_i;
+        t.pr_j = share->pr_j;
+    } else if ((rc = dalloc(E, &t.pr_i, cap)) || (rc = dalloc(E, &t.pr_j, cap))) {
         return rc;
+    }
+    if ((rc = dalloc(E, &t.link, cap))) return rc;
+    HIPCHK(hipMemset(t.link, 0xff, sizeof(int) * (size_t)cap));     // -1: no contact
+    if ((rc = dalloc(E, &t.ci, ccap)) || (rc = dalloc(E, &t.cj, ccap)) ||
+        (rc = dalloc(E, &t.ccounter, kSeg * kCtrStride)))
+        return rc;
+    HIPCHK(hipMemset(t.ccounter, 0, sizeof(int) * kSeg * kCtrStride));
     double** d[] = {&t.xix, &t.xiy, &t.nx, &t.ny, &t.c, &t.fnx, &t.fny, &t.fsx,
                     &t.fsy, &t.bx, &t.by, &t.px, &t.py};
     for (double** q : d)
-        if ((rc = dalloc(E, q, cap))) return rc;
-    if ((rc = dalloc(E, &t.fr, cap))) return rc;
+        if ((rc = dalloc(E, q, ccap))) return rc;
+    if ((rc = dalloc(E, &t.fr, ccap))) return rc;
     if ((rc = dalloc(E, &t.row_start, n)) || (rc = dalloc(E, &t.row_cnt, n)) ||
         (rc = dalloc(E, &t.counter, kSeg * kCtrStride)))
         return rc;
@@ -1765,12 +1915,22 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     }
     // contact tables
     const long long tcap = std::min<long long>(12LL * n + 64LL * 4096, (1LL << kImgShift) - 1);
-    if ((rc = alloc_table(E, E->T[0], (int)tcap, n)) || (rc = alloc_table(E, E->T[1], (int)tcap, n)))
+    // work list: segment s receives the survivors of the listed pairs that
+    // the workgroups with xcd_segment() == s visit: at most G/kSeg + 1
+    // workgroups (G = pair_grid), each with a chunk of at most
+    // ceil(total/G) + kBlock items, total <= kSeg * table segment.  The
+    // contact list has the same bound for the contact kernel's contacts,
+    // plus room for the (rare) history entries carried forward.
+    {
+        const long long G = E->pair_grid, ts = tcap / kSeg;
+        const long long cap = ts + (ts * kSeg + G - 1) / G + (G / kSeg + 2) * kBlock;
+        if (cap + cap / 4 > (1LL << 30) / kSeg) return fail(E, XDEM_E_CAPACITY, "work list too large");
+        E->W.seg_cap = (int)cap;
+    }
+    const int cseg = E->W.seg_cap + E->W.seg_cap / 4;
+    if ((rc = alloc_table(E, E->T[0], (int)tcap, n, nullptr, cseg)) ||
+        (rc = alloc_table(E, E->T[1], (int)tcap, n, &E->T[0], cseg)))
         return rc;
-    // work list: a segment receives the survivors of the listed pairs that
-    // the blocks of that segment visit, at most a table segment plus one
-    // round of kBlock per block
-    E->W.seg_cap = E->T[0].seg_cap + (E->pair_grid / kSeg + 1) * kBlock;
     if ((rc = dalloc(E, &E->W.item, (size_t)E->W.seg_cap * kSeg)) ||
         (rc = dalloc(E, &E->W.counter, kSeg * kCtrStride)))
         return rc;
@@ -1812,7 +1972,8 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
     }
     int acc = 0;
     for (int p = 0; p < E->n; ++p) { start[p] = acc; acc += cnt[p]; }
-    if (acc > E->T[0].seg_cap) return fail(E, XDEM_E_CAPACITY, "too many contacts");
+    if (acc > E->T[0].seg_cap || (long long)acc > (long long)E->T[0].cseg_cap * kSeg)
+        return fail(E, XDEM_E_CAPACITY, "too many contacts");
     // rows in ascending partner ID, the order the step kernels keep
     std::vector<int> order(nc);
     for (int k = 0; k < nc; ++k) order[k] = k;
@@ -1821,18 +1982,16 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
     for (int k : order) slot[k] = start[ii[k]] + fill[ii[k]]++;
     Table& t = E->T[E->cur];
     int rc;
-    int *ci_d, *cj_d, *ej_d, *slot_d;
+    int *ci_d, *cj_d, *slot_d;
     double *a_d, *b_d, *c_d, *d_d, *e_d;
     const size_t base = E->allocs.size();
     if ((rc = dalloc(E, &ci_d, nc)) || (rc = dalloc(E, &cj_d, nc)) || (rc = dalloc(E, &slot_d, nc)) ||
-        (rc = dalloc(E, &ej_d, nc)) ||
         (rc = dalloc(E, &a_d, nc)) || (rc = dalloc(E, &b_d, nc)) || (rc = dalloc(E, &c_d, nc)) ||
         (rc = dalloc(E, &d_d, nc)) || (rc = dalloc(E, &e_d, nc)))
         return rc;
     if (nc > 0) {
         HIPCHK(hipMemcpy(ci_d, ii.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(cj_d, jj.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ej_d, cj, sizeof(int) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(slot_d, slot.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(a_d, xi_x, sizeof(double) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b_d, xi_y, sizeof(double) * nc, hipMemcpyHostToDevice));
@@ -1840,7 +1999,7 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
         HIPCHK(hipMemcpy(d_d, n_y, sizeof(double) * nc, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e_d, damping, sizeof(double) * nc, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_load_contacts, dim3((nc + 255) / 256), dim3(256), 0, E->stream, t,
-                           nc, ci_d, cj_d, ej_d, a_d, b_d, c_d, d_d, e_d, slot_d);
+                           nc, ci_d, cj_d, a_d, b_d, c_d, d_d, e_d, slot_d);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipMemcpyAsync(t.row_start, start.data(), sizeof(int) * E->n, hipMemcpyHostToDevice, E->stream));
@@ -2009,34 +2168,37 @@ int xdem_get_contacts(xdem_engine* E, int cap, int* nc, int* ci, int* cj, double
     const Table& t = E->T[E->cur];
     std::vector<int> seg_used(kSeg, 0);
     if ((rc = segment_counts(E, t, seg_used))) return rc;
-    const int used = t.cap;
-    std::vector<int> pi(used), pj(used);
-    HIPCHK(hipMemcpy(pi.data(), t.pr_i, sizeof(int) * used, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(pj.data(), t.ct_j, sizeof(int) * used, hipMemcpyDeviceToHost));
+    // contacts reachable from the slots in use (what the sums see)
+    const size_t used = (size_t)t.cap, ccap = (size_t)t.cseg_cap * kSeg;
+    std::vector<int> link(used), li(ccap), lj(ccap);
+    HIPCHK(hipMemcpy(link.data(), t.link, sizeof(int) * used, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(li.data(), t.ci, sizeof(int) * ccap, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lj.data(), t.cj, sizeof(int) * ccap, hipMemcpyDeviceToHost));
     std::vector<int> idx;
     idx.reserve(used / 4);
-    for (int e = 0; e < used; ++e) {
-        if (e % t.seg_cap >= seg_used[e / t.seg_cap]) continue;
-        if (pj[e] < 0) continue;
-        pi[e] = E->perm[pi[e]];
-        pj[e] = E->perm[pj[e]];
-        idx.push_back(e);
+    for (size_t e = 0; e < used; ++e) {
+        if ((int)(e % t.seg_cap) >= seg_used[e / t.seg_cap]) continue;
+        const int c = link[e];
+        if (c < 0) continue;
+        li[c] = E->perm[li[c]];
+        lj[c] = E->perm[lj[c]];
+        idx.push_back(c);
     }
     std::sort(idx.begin(), idx.end(), [&](int a, int b) {
-        return pi[a] != pi[b] ? pi[a] < pi[b] : pj[a] < pj[b];
+        return li[a] != li[b] ? li[a] < li[b] : lj[a] < lj[b];
     });
     *nc = (int)idx.size();
     const int m = std::min<int>(cap, (int)idx.size());
-    std::vector<double> buf(used);
+    std::vector<double> buf(ccap);
     auto fetch = [&](double* dst, const double* src) -> int {
         if (!dst) return XDEM_OK;
-        HIPCHK(hipMemcpy(buf.data(), src, sizeof(double) * used, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(buf.data(), src, sizeof(double) * ccap, hipMemcpyDeviceToHost));
         for (int k = 0; k < m; ++k) dst[k] = buf[idx[k]];
         return XDEM_OK;
     };
     for (int k = 0; k < m; ++k) {
-        if (ci) ci[k] = pi[idx[k]];
-        if (cj) cj[k] = pj[idx[k]];
+        if (ci) ci[k] = li[idx[k]];
+        if (cj) cj[k] = lj[idx[k]];
     }
     if ((rc = fetch(fn_x, t.fnx)) || (rc = fetch(fn_y, t.fny)) || (rc = fetch(fs_x, t.fsx)) ||
         (rc = fetch(fs_y, t.fsy)) || (rc = fetch(point_x, t.px)) || (rc = fetch(point_y, t.py)) ||
@@ -2044,8 +2206,8 @@ int xdem_get_contacts(xdem_engine* E, int cap, int* nc, int* ci, int* cj, double
         (rc = fetch(n_y, t.ny)) || (rc = fetch(damping, t.c)))
         return rc;
     if (torque_i || torque_j) {
-        std::vector<FRec> fr(used);
-        HIPCHK(hipMemcpy(fr.data(), t.fr, sizeof(FRec) * used, hipMemcpyDeviceToHost));
+        std::vector<FRec> fr(ccap);
+        HIPCHK(hipMemcpy(fr.data(), t.fr, sizeof(FRec) * ccap, hipMemcpyDeviceToHost));
         for (int k = 0; k < m; ++k) {
             if (torque_i) torque_i[k] = fr[idx[k]].t1;
             if (torque_j) torque_j[k] = fr[idx[k]].t2;
@@ -2135,11 +2297,11 @@ int xdem_get_traffic(xdem_engine* E, double* bytes_per_step, long long* contacts
     const Table& t = E->T[E->cur];
     std::vector<int> seg_used(kSeg, 0);
     if ((rc = segment_counts(E, t, seg_used))) return rc;
-    std::vector<int> pj(t.cap);
-    HIPCHK(hipMemcpy(pj.data(), t.ct_j, sizeof(int) * (size_t)t.cap, hipMemcpyDeviceToHost));
+    std::vector<int> link(t.cap);
+    HIPCHK(hipMemcpy(link.data(), t.link, sizeof(int) * (size_t)t.cap, hipMemcpyDeviceToHost));
     long long live = 0;
     for (int e = 0; e < t.cap; ++e)
-        if (e % t.seg_cap < seg_used[e / t.seg_cap] && pj[e] >= 0) ++live;
+        if (e % t.seg_cap < seg_used[e / t.seg_cap] && link[e] >= 0) ++live;
     // state 96 B r/w + static 44 B per particle; 44 B history r/w per contact
     if (bytes_per_step) *bytes_per_step = 140.0 * E->n + 88.0 * (double)live;
     if (contacts) *contacts = live;

@@ -56,6 +56,8 @@ def lib():
         L.oracle_pair_root.argtypes = [C.c_double] * 10 + [_D]
         L.oracle_newton.argtypes = [C.c_double] * 6 + [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
         L.oracle_bisect.argtypes = [C.c_double] * 6 + [C.POINTER(C.c_double)]
+        L.oracle_set_control.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6
+        L.oracle_get_box.argtypes = [C.c_void_p, _D]
         _lib = L
     return _lib
 
@@ -133,6 +135,20 @@ class OracleWorld:
     def boundary(self):
         out = np.zeros(3)
         lib().oracle_get_boundary(self.h, out)
+        return out
+
+    def set_control(self, mode, stress=0.0, rate_x=0.0, rate_y=0.0, rattler_scale=1.0,
+                    reset_rattlers=False, boundary_mass=0.0, max_strain_rate=0.0):
+        """Boundary control applied inside every step (see xdem_set_control)."""
+        self._check(lib().oracle_set_control(self.h, int(mode), int(bool(reset_rattlers)),
+                                             float(rattler_scale), float(stress),
+                                             float(rate_x), float(rate_y),
+                                             float(boundary_mass), float(max_strain_rate)))
+
+    def box(self):
+        """xmin, xmax, ymin, ymax, boundary velocity x, y, last control stress (4)."""
+        out = np.zeros(10)
+        lib().oracle_get_box(self.h, out)
         return out
 
     def counters(self):
@@ -222,6 +238,43 @@ def read_golden(path):
         d["boundary"] = take(np.float64, 3)
         steps.append(d)
     return steps
+
+
+def read_control_golden(path):
+    """Parse a ref_harness `control` dump: per step the `run` record plus
+    `box` = (xmin, xmax, ymin, ymax, vel x, vel y, control stress xx, xy, yx, yy)."""
+    buf = open(path, "rb").read()
+    assert buf[:8] == b"XDEMGC01"
+    tmp = path + ".gd"
+    steps = []
+    off = 8
+    # records are `run` records followed by 10 doubles; split by re-reading
+    body = buf[8:]
+    pos = 0
+    while pos < len(body):
+        n = int(np.frombuffer(body, np.int32, 2, pos)[1])
+        p = pos + 8 + 9 * 8 * n + 4 * n
+        nc = int(np.frombuffer(body, np.int32, 1, p)[0])
+        end = p + 4 + 8 * nc + 13 * 8 * nc + 4 * 8 + 3 * 8
+        with open(tmp, "wb") as f:
+            f.write(b"XDEMGD01" + body[pos:end])
+        d = read_golden(tmp)[0]
+        d["box"] = np.frombuffer(body, np.float64, 10, end).copy()
+        steps.append(d)
+        pos = end + 80
+    os.remove(tmp)
+    return steps
+
+
+def run_reference_control(state_path, out_path, nsteps, mode, stress=0.0, rate_x=0.0,
+                          rate_y=0.0, rattler_scale=1.0, reset_rattlers=False,
+                          boundary_mass=0.0, max_strain_rate=0.0):
+    subprocess.run([HARNESS, "control", state_path, out_path, str(nsteps), str(mode),
+                    repr(float(stress)), repr(float(rate_x)), repr(float(rate_y)),
+                    repr(float(rattler_scale)), str(int(bool(reset_rattlers))),
+                    repr(float(boundary_mass)), repr(float(max_strain_rate))],
+                   check=True, stdout=subprocess.DEVNULL, timeout=300)
+    return read_control_golden(out_path)
 
 
 def run_reference(state_path, out_path, nsteps, every=1, threads=1):

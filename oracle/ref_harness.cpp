@@ -11,6 +11,13 @@
 //       on a list of poses (src/EllipticParticle.cpp:16-32).
 //   ref_harness bench <state.xst> <warmup> <steps> <threads> <chunk>
 //       wall-clock timing of takeTimeStep, prints one JSON line.
+//   ref_harness control <state.xst> <out.xdg> <nsteps> <mode> <stress> <rate_x>
+//                       <rate_y> <rattler_scale> <reset_rattlers> <mass> <max_rate>
+//       a PeriodicControlWorld run in the order of the drivers' compression
+//       loop (mainIsoElpsPack_WCCM.cpp:127-137): scaleVelocityOfRattlers, the
+//       step, resetVelocityOfRattlers, updateTotalStress, boundary update
+//       (mode 1 strain, 2 stress uniform, 3 stress X then Y, 4 stress uniform
+//       with strain limit); dumps every step plus the boundary record.
 //
 // The state file layout is written by xdem_amd/state.py (write_state).
 #include <chrono>
@@ -27,6 +34,7 @@
 #include "CircularParticle.h"
 #include "GluedParticle.h"
 #include "PeriodicBoundaryWorld.h"
+#include "PeriodicControlWorld.h"
 #include "LeesEdwardsBoundaryWorld.h"
 
 namespace {
@@ -110,6 +118,15 @@ struct HLE : HWorld<LeesEdwardsBoundaryWorld> {
     void scalars(double* o) const { o[0] = _totalShift; o[1] = _shiftVel; o[2] = _totalStrain; }
 };
 using HPB = HWorld<PeriodicBoundaryWorld>;
+struct HPC : HWorld<PeriodicControlWorld> {
+    using HWorld::HWorld;
+    // xmin, xmax, ymin, ymax, boundary velocity x, y, total stress
+    void box(double* o) const {
+        o[0] = _xmin; o[1] = _xmax; o[2] = _ymin; o[3] = _ymax; o[4] = _velX; o[5] = _velY;
+        o[6] = _totalStress.xx(); o[7] = _totalStress.xy();
+        o[8] = _totalStress.yx(); o[9] = _totalStress.yy();
+    }
+};
 
 struct Built {
     State s;
@@ -118,10 +135,13 @@ struct Built {
     vector<BaseParticle*> handles;
     std::unique_ptr<HLE> le;
     std::unique_ptr<HPB> pb;
-    BaseWorld* world() { return le ? (BaseWorld*)le.get() : (BaseWorld*)pb.get(); }
+    std::unique_ptr<HPC> pc;
+    BaseWorld* world() {
+        return le ? (BaseWorld*)le.get() : pc ? (BaseWorld*)pc.get() : (BaseWorld*)pb.get();
+    }
 };
 
-void build(Built& B, int threads, int chunk) {
+void build(Built& B, int threads, int chunk, bool control = false) {
     State& s = B.s;
     const double* c = s.cfg;
     B.cm.reset(new ContactModel(c[5], c[6], c[7], c[8]));
@@ -168,6 +188,9 @@ void build(Built& B, int threads, int chunk) {
         B.le->setWorldBoundary(c[0], c[1], c[2], c[3]);
         B.le->setShiftStrainRate(c[10]);
         B.le->set_shift(c[11], c[12]);
+    } else if (control) {
+        B.pc.reset(new HPC(c[4], B.handles, B.cm.get()));
+        B.pc->setWorldBoundary(c[0], c[1], c[2], c[3]);
     } else {
         B.pb.reset(new HPB(c[4], B.handles, B.cm.get()));
         B.pb->setWorldBoundary(c[0], c[1], c[2], c[3]);
@@ -193,6 +216,7 @@ void dump(Built& B, FILE* f, int step) {
     }
     const vector<vector2d>* F; const vector<double>* T; const vector<int>* NC;
     if (B.le) { F = &B.le->force(); T = &B.le->torque(); NC = &B.le->ncont(); }
+    else if (B.pc) { F = &B.pc->force(); T = &B.pc->torque(); NC = &B.pc->ncont(); }
     else { F = &B.pb->force(); T = &B.pb->torque(); NC = &B.pb->ncont(); }
     for (int i = 0; i < n; ++i) a[i] = (*F)[i].x();
     wr(f, a.data(), n);
@@ -310,6 +334,48 @@ int mode_bench(int argc, char** argv) {
     return 0;
 }
 
+int mode_control(int argc, char** argv) {
+    if (argc < 13) return 2;
+    Built B;
+    B.s = read_state(argv[2]);
+    const int nsteps = atoi(argv[4]), mode = atoi(argv[5]);
+    const double stress = atof(argv[6]), rx = atof(argv[7]), ry = atof(argv[8]);
+    const double rscale = atof(argv[9]);
+    const int reset = atoi(argv[10]);
+    const double mass = atof(argv[11]), maxrate = atof(argv[12]);
+    build(B, 1, 100, true);
+    HPC& W = *B.pc;
+    W.updateTotalParticleVolume();
+    if (mass > 0) W.setVirtualBoundaryMass(mass);
+    else W.setVirtualBoundaryMass();
+    W.setMaxStrainRate(maxrate);
+    FILE* f = fopen(argv[3], "wb");
+    if (!f) return 3;
+    fwrite("XDEMGC01", 1, 8, f);
+    for (int s = 1; s <= nsteps; ++s) {
+        if (rscale != 1.0) W.scaleVelocityOfRattlers(rscale);
+        W.modifyParticlePosition();
+        W.findPossibleContacts();
+        W.updateContacts();
+        W.collectForceAndTorque();
+        W.takeTimeIntegral();
+        if (reset) W.resetVelocityOfRattlers();
+        W.updateTotalStress();
+        if (mode == 1) W.updatePeriodicBoundary_strainControl(rx, ry);
+        else if (mode == 2) W.updatePeriodicBoundary_stressControlUniform(stress);
+        else if (mode == 3) {
+            W.updatePeriodicBoundary_stressControlX(stress);
+            W.updatePeriodicBoundary_stressControlY(stress);
+        } else if (mode == 4) W.updatePeriodicBoundary_stressControlUniform_strainLimit(stress);
+        double bx[10];
+        W.box(bx);
+        dump(B, f, s);
+        wr(f, bx, 10);
+    }
+    fclose(f);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -319,5 +385,6 @@ int main(int argc, char** argv) {
     if (m == "run") return mode_run(argc, argv);
     if (m == "pairs") return mode_pairs(argc, argv);
     if (m == "bench") return mode_bench(argc, argv);
+    if (m == "control") return mode_control(argc, argv);
     return 2;
 }

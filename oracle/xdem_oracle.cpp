@@ -100,6 +100,14 @@ struct World {
     // counters
     long long newton_iters = 0, candidates = 0, coarse = 0, steps = 0, rescues = 0;
     std::string err;
+    // PeriodicControlWorld boundary control and the rattler treatment of the
+    // drivers (src/PeriodicControlWorld.h:8-11, src/BaseParticle.h:35)
+    int ctl_mode = 0, ctl_reset = 0;
+    double ctl_rscale = 1.0, ctl_stress = 0.0, ctl_rx = 0.0, ctl_ry = 0.0;
+    double bmass = 0.0, max_rate = 0.0, velx = 0.0, vely = 0.0;
+    double tstress[4] = {0, 0, 0, 0};
+    std::vector<char> rattler;
+    double volume = 0.0;                     // _totalParticleVolume
 };
 
 // ---------------------------------------------------------------------------
@@ -474,9 +482,119 @@ static int contact_force(World& W, int i, int j, double ox, double oy,
 }
 
 // one takeTimeStep of either world.
+// BaseParticle::isRattlerParticle (src/BaseParticle.cpp:153-156); a clump is
+// a rattler when all its sub-particles are (src/GluedParticle.cpp:141-147).
+static bool is_rattler(const World& W, int i) {
+    if (W.kind[i] != KIND_CLUMP) return W.rattler[i] != 0;
+    for (int s = i + 1; s < W.n && W.up[s] == i; ++s)
+        if (!W.rattler[s]) return false;
+    return true;
+}
+
+// BaseWorld::updateTotalStress, src/BaseWorld.cpp:158-176, with
+// BaseParticle::getStressFromContactList (src/BaseParticle.cpp:143-151) and
+// GluedParticle::getStressFromContactList (src/GluedParticle.cpp:127-139).
+static void total_stress(const World& W, double* out) {
+    double sxx = 0, sxy = 0, syx = 0, syy = 0;
+    for (int p = 0; p < W.n; ++p) {
+        if (W.flags[p] & SKIP_MACRO_SUM) continue;
+        double txx = 0, txy = 0, tyx = 0, tyy = 0;
+        if (W.kind[p] == KIND_CLUMP) {
+            for (int s = p + 1; s < W.n && W.up[s] == p; ++s) {
+                for (auto& kv : W.hist[s]) {
+                    const Contact& c = kv.second;
+                    double bx = c.bx + W.cbx[s], by = c.by + W.cby[s];
+                    txx += c.fx * bx; txy += c.fx * by; tyx += c.fy * bx; tyy += c.fy * by;
+                }
+            }
+        } else {
+            for (auto& kv : W.hist[p]) {
+                const Contact& c = kv.second;
+                txx += c.fx * c.bx; txy += c.fx * c.by; tyx += c.fy * c.bx; tyy += c.fy * c.by;
+            }
+        }
+        sxx += txx; sxy += txy; syx += tyx; syy += tyy;
+    }
+    double area = W.xlen * W.ylen;
+    out[0] = sxx / area; out[1] = sxy / area; out[2] = syx / area; out[3] = syy / area;
+}
+
+// BaseWorld::scaleAllParticlePosition, src/BaseWorld.cpp:119-124
+static void scale_positions(World& W, double sx, double sy) {
+    for (int i = 0; i < W.n; ++i) { W.x[i] *= sx; W.y[i] *= sy; }
+}
+
+// The boundary updates of PeriodicControlWorld, src/PeriodicControlWorld.cpp:124-237
+// (MAX_SCALE_RATE, :7).  Each uses the stress of the last updateTotalStress.
+static const double kMaxScaleRate = 100000000.0;
+
+static void control_update(World& W) {
+    const double dt = W.dt;
+    if (W.ctl_mode == 1) {                       // :124-135
+        double scaleX = 1.0 + W.ctl_rx * W.xlen * dt;
+        double scaleY = 1.0 + W.ctl_ry * W.ylen * dt;
+        W.xmin *= scaleX; W.xmax *= scaleX;
+        W.ymin *= scaleY; W.ymax *= scaleY;
+        W.xlen = W.xmax - W.xmin; W.ylen = W.ymax - W.ymin;
+        scale_positions(W, scaleX, scaleY);
+    } else if (W.ctl_mode == 2) {                // :137-151
+        double hydro = (W.tstress[0] + W.tstress[3]) / 2.0;
+        double deltaVel = (W.ctl_stress - hydro) * W.xlen * dt;
+        W.velx += deltaVel;
+        W.vely = W.velx;
+        double scale = (W.xmin + deltaVel * dt) / W.xmin;
+        W.xmin += deltaVel * dt; W.xmax = -W.xmin; W.xlen = W.xmax - W.xmin;
+        W.ymin += deltaVel * dt; W.ymax = -W.ymin; W.ylen = W.ymax - W.ymin;
+        scale_positions(W, scale, scale);
+    } else if (W.ctl_mode == 3) {                // :153-193, X then Y
+        {
+            double xminPre = W.xmin;
+            double stressX = W.tstress[0];
+            if (std::fabs(stressX) < W.ctl_stress) W.velx *= (1.0 - dt * 1.0);
+            double deltaVel = (W.ctl_stress - stressX) * W.xlen / W.bmass * dt;
+            W.velx = deltaVel;
+            if (std::fabs(W.velx) / W.xlen > kMaxScaleRate * dt) W.velx = kMaxScaleRate * dt * W.xlen;
+            W.xmin += W.velx * dt; W.xmax = -W.xmin; W.xlen = W.xmax - W.xmin;
+            double scale = W.xmin / xminPre;
+            scale_positions(W, scale, 1.0);
+        }
+        {
+            double yminPre = W.ymin;
+            double stressY = W.tstress[3];
+            if (std::fabs(stressY) < W.ctl_stress) W.vely *= (1.0 - dt * 1.0);
+            double deltaVel = (W.ctl_stress - stressY) * W.ylen / W.bmass * dt;
+            W.vely = deltaVel;
+            if (std::fabs(W.vely) / W.ylen > kMaxScaleRate * dt) W.vely = kMaxScaleRate * dt * W.ylen;
+            W.ymin += W.vely * dt; W.ymax = -W.ymin; W.ylen = W.ymax - W.ymin;
+            double scale = W.ymin / yminPre;
+            scale_positions(W, 1.0, scale);
+        }
+    } else if (W.ctl_mode == 4) {                // :213-237
+        double xminPre = W.xmin;
+        double hydro = (W.tstress[0] + W.tstress[3]) / 2.0;
+        double deltaVel = (W.ctl_stress - hydro) * W.xlen * dt;
+        W.velx += deltaVel;
+        if ((std::fabs(W.velx) / (W.xlen / 2.0)) > W.max_rate)
+            W.velx = std::fabs(W.velx) / W.velx * (W.xlen / 2.0) * W.max_rate;
+        W.vely = W.velx;
+        W.xmin += W.velx * dt; W.xmax = -W.xmin; W.xlen = W.xmax - W.xmin;
+        W.ymin += W.vely * dt; W.ymax = -W.ymin; W.ylen = W.ymax - W.ymin;
+        double scale = W.xmin / xminPre;
+        scale_positions(W, scale, scale);
+    }
+}
+
 static int step_once(World& W) {
     const int n = W.n;
     const bool le = W.world_kind == WORLD_LE;
+    // scaleVelocityOfRattlers, src/BaseWorld.cpp:94-106 (the drivers call it
+    // before the step, mainIsoElpsPack_WCCM.cpp:129)
+    if (W.ctl_rscale != 1.0) {
+        for (int i = 0; i < n; ++i) {
+            if (W.flags[i] & SKIP_MACRO_SUM) continue;
+            if (is_rattler(W, i)) { W.vx[i] *= W.ctl_rscale; W.vy[i] *= W.ctl_rscale; W.w[i] *= W.ctl_rscale; }
+        }
+    }
     // modifyParticlePosition: LE src/LeesEdwardsBoundaryWorld.cpp:304-333,
     // periodic src/PeriodicBoundaryWorld.cpp:129-142
     for (int i = 0; i < n; ++i) {
@@ -620,6 +738,9 @@ static int step_once(World& W) {
             W.ncont[p1]++; W.ncont[p2]++;
         }
     }
+    // setAllParticlesAsRattler, then setAsNonRattler for both partners of
+    // every entry (src/BaseWorld.cpp:25, 44-45)
+    for (int i = 0; i < n; ++i) W.rattler[i] = W.ncont[i] == 0 ? 1 : 0;
     // takeTimeIntegral, src/BaseWorld.cpp:59-77 -> src/BaseParticle.cpp:107-118,
     // clumps src/GluedParticle.cpp:91-125
     const double g = W.gdamp, dt = W.dt;
@@ -660,6 +781,20 @@ static int step_once(World& W) {
                 W.th[k] += ang;
             }
         }
+    }
+    // resetVelocityOfRattlers, src/BaseWorld.cpp:79-92
+    // (PeriodicControlWorld::takeTimeStep, src/PeriodicControlWorld.cpp:114-122)
+    if (W.ctl_reset) {
+        for (int i = 0; i < n; ++i) {
+            if (W.flags[i] & SKIP_MACRO_SUM) continue;
+            if (is_rattler(W, i)) { W.vx[i] = 0.0; W.vy[i] = 0.0; W.w[i] = 0.0; }
+        }
+    }
+    // updateTotalStress and the boundary update of the compression loop
+    // (mainIsoElpsPack_WCCM.cpp:135-137)
+    if (W.ctl_mode != 0) {
+        total_stress(W, W.tstress);
+        control_update(W);
     }
     // updateLeesEdwardBoundary, src/LeesEdwardsBoundaryWorld.cpp:285-296
     if (le) {
@@ -732,6 +867,7 @@ int oracle_set_particles(void* h, int n, const int* kind, const int* up,
     if (cby) W.cby.assign(cby, cby + n);
     W.hist.assign(n, {});
     W.fx.assign(n, 0.0); W.fy.assign(n, 0.0); W.tq.assign(n, 0.0); W.ncont.assign(n, 0);
+    W.rattler.assign(n, 0);
     // updateMaximumParticleSize, src/BaseWorld.cpp:201-219
     if (!(W.cell > 0)) {
         double mx = 0.0;
@@ -827,29 +963,40 @@ int oracle_get_contacts(void* h, int cap, int* nc, int* ci, int* cj,
 // BaseParticle::getStressFromContactList (src/BaseParticle.cpp:143-151) and
 // GluedParticle::getStressFromContactList (src/GluedParticle.cpp:127-139).
 int oracle_get_stress(void* h, double* out) {
+    total_stress(*(World*)h, out);
+    return XDEM_OK;
+}
+
+// Boundary control, mirrors xdem_set_control: mode 1 strain, 2 stress
+// uniform, 3 stress X then Y, 4 stress uniform with strain limit.
+// mass <= 0: setVirtualBoundaryMass() (src/PeriodicControlWorld.cpp:58-66).
+int oracle_set_control(void* h, int mode, int reset_rattlers, double rattler_scale,
+                       double stress, double rate_x, double rate_y, double mass,
+                       double max_rate) {
     World& W = *(World*)h;
-    double sxx = 0, sxy = 0, syx = 0, syy = 0;
-    for (int p = 0; p < W.n; ++p) {
-        if (W.flags[p] & SKIP_MACRO_SUM) continue;
-        double txx = 0, txy = 0, tyx = 0, tyy = 0;
-        if (W.kind[p] == KIND_CLUMP) {
-            for (int s = p + 1; s < W.n && W.up[s] == p; ++s) {
-                for (auto& kv : W.hist[s]) {
-                    const Contact& c = kv.second;
-                    double bx = c.bx + W.cbx[s], by = c.by + W.cby[s];
-                    txx += c.fx * bx; txy += c.fx * by; tyx += c.fy * bx; tyy += c.fy * by;
-                }
-            }
-        } else {
-            for (auto& kv : W.hist[p]) {
-                const Contact& c = kv.second;
-                txx += c.fx * c.bx; txy += c.fx * c.by; tyx += c.fy * c.bx; tyy += c.fy * c.by;
-            }
+    if (mode < 0 || mode > 4 || (mode != 0 && W.world_kind != WORLD_PERIODIC))
+        return XDEM_E_INVALID;
+    W.ctl_mode = mode; W.ctl_reset = reset_rattlers; W.ctl_rscale = rattler_scale;
+    W.ctl_stress = stress; W.ctl_rx = rate_x; W.ctl_ry = rate_y; W.max_rate = max_rate;
+    if (mass > 0) {
+        W.bmass = mass;
+    } else {
+        double m = 0.0;
+        for (int i = 0; i < W.n; ++i) {
+            if (W.flags[i] & SKIP_MACRO_SUM) continue;
+            m += W.m[i];
         }
-        sxx += txx; sxy += txy; syx += tyx; syy += tyy;
+        W.bmass = m / (double(W.n) * 10.0);
     }
-    double area = W.xlen * W.ylen;
-    out[0] = sxx / area; out[1] = sxy / area; out[2] = syx / area; out[3] = syy / area;
+    return XDEM_OK;
+}
+
+// xmin, xmax, ymin, ymax, boundary velocity x, y, stress of the last control
+int oracle_get_box(void* h, double* out) {
+    World& W = *(World*)h;
+    out[0] = W.xmin; out[1] = W.xmax; out[2] = W.ymin; out[3] = W.ymax;
+    out[4] = W.velx; out[5] = W.vely;
+    for (int k = 0; k < 4; ++k) out[6 + k] = W.tstress[k];
     return XDEM_OK;
 }
 

@@ -11,6 +11,12 @@ Fixtures
   world_<case>.npz  initial state (+ optional history) and the reference's
                     particle state, forces, contacts, stress and LE scalars
                     after steps 1, 2, 4 and 8 of takeTimeStep (NUM_THREADS=1)
+  control_<case>.npz  a PeriodicControlWorld run in the order of the drivers'
+                    compression loop (scaleVelocityOfRattlers, step,
+                    resetVelocityOfRattlers, updateTotalStress, boundary
+                    update; src/PeriodicControlWorld.cpp:124-237): state,
+                    contacts, stress and the boundary record after steps
+                    1, 5 and 20
   pairs.npz         calculateContactGeometry of the reference on random
                     ellipse and circle poses (src/EllipticParticle.cpp:16-285,
                     src/CircularParticle.cpp:67-92)
@@ -63,6 +69,38 @@ def run_case(name, tmp):
     print(f"{name}: n={s.n} contacts after {STEPS} steps={steps[STEPS]['i'].size}")
 
 
+# name: (packing, mode, control arguments)
+CONTROL_CASES = {
+    "stress_xy": ("pb_ellipse", 3, dict(stress=5.0, rattler_scale=0.99)),
+    "strain": ("pb_circle", 1, dict(rate_x=-0.5, rate_y=-0.3)),
+    "stress_uniform": ("pb_ar1", 2, dict(stress=8.0, reset_rattlers=True)),
+    "stress_limited": ("pb_ellipse", 4, dict(stress=20.0, max_strain_rate=0.05)),
+    "clumps": ("pb_clumps", 2, dict(stress=2.0, rattler_scale=0.99, reset_rattlers=True)),
+}
+CONTROL_STEPS = 20
+CONTROL_DUMP = (1, 5, 20)
+
+
+def run_control(name, tmp):
+    base, mode, kw = CONTROL_CASES[name]
+    s = helpers.state(base)
+    sp = os.path.join(tmp, f"ctl_{name}.xst")
+    write_state(sp, s)
+    steps = po.run_reference_control(sp, os.path.join(tmp, f"ctl_{name}.bin"), CONTROL_STEPS,
+                                     mode, **kw)
+    out = {"in_" + k: v for k, v in state_arrays(s).items()}
+    out["mode"] = np.array([mode])
+    for k in ("stress", "rate_x", "rate_y", "rattler_scale", "reset_rattlers",
+              "boundary_mass", "max_strain_rate"):
+        out["arg_" + k] = np.array([float(kw.get(k, 1.0 if k == "rattler_scale" else 0.0))])
+    for st in CONTROL_DUMP:
+        for k, v in steps[st - 1].items():
+            if k != "step":
+                out[f"s{st}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, f"control_{name}.npz"), **out)
+    print(f"control {name}: n={s.n} box after {CONTROL_STEPS} steps {steps[-1]['box'][:4]}")
+
+
 def run_pairs(tmp, m=3000, seed=42):
     rng = np.random.default_rng(seed)
     rows = np.zeros((m, 12))
@@ -92,6 +130,8 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         for c in CASES:
             run_case(c, tmp)
+        for c in CONTROL_CASES:
+            run_control(c, tmp)
         run_pairs(tmp)
 
 

@@ -29,3 +29,26 @@ def load_world(name):
 def load_pairs():
     z = np.load(os.path.join(GOLDEN, "pairs.npz"))
     return z["poses"], z["geometry"]
+
+
+CONTROL_CASES = ("stress_xy", "strain", "stress_uniform", "stress_limited", "clumps")
+CONTROL_DUMP = (1, 5, 20)
+
+
+def load_control(name):
+    """(state, mode, control keyword arguments, {step: reference dump})."""
+    z = np.load(os.path.join(GOLDEN, f"control_{name}.npz"))
+    cfgv = z["in_cfg"]
+    cfg = WorldConfig(world_kind=int(cfgv[0]), **{k: float(v) for k, v in zip(_CFG_KEYS, cfgv[1:])})
+    f = lambda k: z["in_" + k].copy()
+    s = State(cfg, *(f(k) for k in ("kind", "up_level", "flags", "r", "aspect", "mass",
+                                     "inertia", "x", "y", "theta", "vx", "vy", "omega",
+                                     "clump_bx", "clump_by", "ci", "cj", "xi_x", "xi_y",
+                                     "n_x", "n_y", "damping")))
+    kw = {k[4:]: float(z[k][0]) for k in z.files if k.startswith("arg_")}
+    kw["reset_rattlers"] = bool(kw["reset_rattlers"])
+    steps = {}
+    for st in CONTROL_DUMP:
+        pre = f"s{st}_"
+        steps[st] = {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
+    return s, int(z["mode"][0]), kw, steps

@@ -64,3 +64,36 @@ def test_world_against_reference(name):
         sg = g["stress"]
         assert float(np.max(np.abs(w.stress() - sg))) <= 1e-12 * float(np.max(np.abs(sg)))
         assert np.array_equal(w.boundary(), g["boundary"]), f"{name}: LE scalars step {st}"
+
+
+@pytest.mark.parametrize("name", __import__("tests.golden_io", fromlist=["x"]).CONTROL_CASES)
+def test_control_against_reference(name):
+    """PeriodicControlWorld boundary control and the drivers' rattler
+    treatment (src/PeriodicControlWorld.cpp:124-237, src/BaseWorld.cpp:79-124)
+    against the real reference over 20 steps: identical contact sets, the
+    boundary and its velocity to a few ulps (the stress they follow is a sum
+    over an unordered_map in the reference), positions and velocities to
+    1e-12."""
+    from tests.golden_io import CONTROL_DUMP, load_control
+    s, mode, kw, steps = load_control(name)
+    w = po.OracleWorld(s)
+    w.set_control(mode, **kw)
+    done = 0
+    for st in CONTROL_DUMP:
+        w.step(st - done)
+        done = st
+        g = steps[st]
+        c, p, b = w.contacts(), w.particles(), w.box()
+        assert np.array_equal(c["i"], g["i"]) and np.array_equal(c["j"], g["j"]), \
+            f"{name}: contact set differs at step {st}"
+        box = g["box"]
+        for k in range(6):
+            assert abs(b[k] - box[k]) <= 1e-13 * max(abs(box[k]), 1e-300), f"{name}: box[{k}] step {st}"
+        assert float(np.max(np.abs(b[6:] - box[6:]))) <= 1e-12 * float(np.max(np.abs(box[6:])))
+        for k in ("x", "y", "theta"):
+            assert float(np.max(np.abs(p[k] - g[k]))) <= 1e-12, f"{name}: {k} step {st}"
+        for k in ("vx", "vy", "omega"):
+            scale = max(float(np.max(np.abs(g[k]))), 1e-300)
+            assert float(np.max(np.abs(p[k] - g[k]))) <= 1e-11 * scale, f"{name}: {k} step {st}"
+        if mode != 1:
+            assert np.max(np.abs(box[4:6])) > 0, "the boundary should move"

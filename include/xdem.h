@@ -177,6 +177,48 @@ int xdem_get_timing(xdem_engine* e, double* contact_ms, long long* launches);
  * history read and write) and the live contact count. */
 int xdem_get_traffic(xdem_engine* e, double* bytes_per_step, long long* contacts);
 
+/* ---- Boundary control (PeriodicControlWorld) -----------------------------
+ * The box control of PeriodicControlWorld (src/PeriodicControlWorld.cpp)
+ * and the rattler treatment the drivers apply around a step, applied inside
+ * every xdem_step of the packing, on the device, in the order of the
+ * drivers' compression loop (mainIsoElpsPack_WCCM.cpp:127-137):
+ *   scaleVelocityOfRattlers(rattler_scale)     src/BaseWorld.cpp:94-106
+ *   the step (modifyParticlePosition ... takeTimeIntegral)
+ *   resetVelocityOfRattlers()                  src/BaseWorld.cpp:79-92, as in
+ *                                              PeriodicControlWorld::takeTimeStep
+ *                                              (src/PeriodicControlWorld.cpp:114-122)
+ *   updateTotalStress()                        src/BaseWorld.cpp:158-176
+ *   the boundary update of `mode`, which moves the box and rescales every
+ *   position (scaleAllParticlePosition, src/BaseWorld.cpp:119-124).
+ * Periodic packings only; the box must be centred on the origin, as the
+ * reference's updates assume (xmax = -xmin after each update). */
+#define XDEM_CONTROL_NONE            0
+#define XDEM_CONTROL_STRAIN          1  /* updatePeriodicBoundary_strainControl(rate_x, rate_y), :124-135 */
+#define XDEM_CONTROL_STRESS_UNIFORM  2  /* updatePeriodicBoundary_stressControlUniform(stress), :137-151 */
+#define XDEM_CONTROL_STRESS_XY       3  /* updatePeriodicBoundary_stressControlX(stress), then ..Y(stress), :153-193 */
+#define XDEM_CONTROL_STRESS_LIMITED  4  /* updatePeriodicBoundary_stressControlUniform_strainLimit(stress), :213-237 */
+
+typedef struct xdem_control {
+    int    mode;             /* XDEM_CONTROL_* */
+    int    reset_rattlers;   /* 1: resetVelocityOfRattlers after the integration */
+    double rattler_scale;    /* scaleVelocityOfRattlers(rattler_scale) before the step; 1: none */
+    double stress;           /* target stress of the stress-controlled modes */
+    double strain_rate_x;    /* strain control rates */
+    double strain_rate_y;
+    double boundary_mass;    /* setVirtualBoundaryMass(mass); <= 0: setVirtualBoundaryMass()
+                                (src/PeriodicControlWorld.cpp:58-66) */
+    double max_strain_rate;  /* setMaxStrainRate (src/PeriodicControlWorld.cpp:73-76) */
+} xdem_control;
+
+/* Configure packing `packing` (replaces calling the methods above between
+ * steps).  The boundary velocities start at zero, as in the reference. */
+int xdem_set_control(xdem_engine* e, int packing, const xdem_control* c);
+
+/* Box of each packing: out[10k..10k+9] = xmin, xmax, ymin, ymax, boundary
+ * velocity x, y (_velX, _velY) and the total stress xx, xy, yx, yy the last
+ * control step followed (src/PeriodicControlWorld.h:8-11, src/BaseWorld.h). */
+int xdem_get_box(xdem_engine* e, double* out);
+
 /* ---- Learned quartic-root guess (XDEM_PREC_NN) --------------------------
  * No counterpart in the reference, whose Newton iteration always starts at
  * s = sqrt(delta + 1) (NewtonRaphsonForQuartic, src/EllipticParticle.cpp:163,

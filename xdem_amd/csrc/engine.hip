@@ -62,6 +62,16 @@ struct Pack {
     double vlenx, vleny;          // list grid: cells >= 2 r_max + skin
     int nx, ny, bin_base, world, p_begin, p_end, newton_cap, strict;
     int vnx, vny;                 // list grid size; bins hold (vny+2)*vnx cells
+    // boundary control (PeriodicControlWorld) and rattler treatment
+    int ctl_mode, ctl_reset;      // XDEM_CONTROL_*; resetVelocityOfRattlers
+    int vnx_alloc, vny_alloc;     // list-grid bins allocated for this packing
+    double ctl_rscale;            // scaleVelocityOfRattlers factor (1: none)
+    double ctl_stress, ctl_rx, ctl_ry, bmass, max_rate;
+    double velx, vely;            // boundary velocities (_velX, _velY)
+    double tstress[4];            // stress of the last control step
+    double scale_x, scale_y;      // this step's position scaling
+    double cell, vcell, rmax;     // reference-grid cell; smallest list-grid cell
+    double sigma;                 // product of the shrink factors since the rebuild
 };
 
 // Per-particle ellipse quantities.  Fixed: b = a/AR, e^2, b^2, b/a - 1 and
@@ -105,6 +115,8 @@ struct Parts {
     int2* in_ref;
     double *dsx, *dsy;            // displacement since the list rebuild (a
                                   // path-length bound for clump subs in dsx)
+    int8_t* rat;                  // BaseParticle::_isRattler of the last step
+    unsigned long long* maxdisp;  // per packing: bits of max 2|displacement|
     int n;
 };
 
@@ -275,6 +287,17 @@ __device__ __forceinline__ void copy_contact(const Table& from, int a, const Tab
     to.px[b] = from.px[a]; to.py[b] = from.py[a];
 }
 
+// BaseParticle::isRattlerParticle (src/BaseParticle.cpp:153-156); a clump is
+// a rattler when all its sub-particles (stored right after it) are
+// (src/GluedParticle.cpp:141-147).
+__device__ __forceinline__ bool is_rattler(const Parts& P, int p) {
+    if (P.kind[p] != KIND_CLUMP) return P.rat[p] != 0;
+    const int ep = P.eid[p];
+    for (int s = p + 1; s < P.n && P.up[s] == ep; ++s)
+        if (!P.rat[s]) return false;
+    return true;
+}
+
 // Image offset of P2 for a pair found in bin row `img`:
 // 0: minimum image (src/PeriodicBoundaryWorld.cpp:97-101,
 //    src/LeesEdwardsBoundaryWorld.cpp:218-222);
@@ -319,6 +342,16 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
     if (p >= P.n || *err) return;
     const bool rb = *rebuild != 0;
     if (!rb && p < kSeg) new_t.counter[p * kCtrStride] = old_t.counter[p * kCtrStride];
+    {
+        // scaleVelocityOfRattlers (src/BaseWorld.cpp:94-106), which the
+        // drivers call before each step (mainIsoElpsPack_WCCM.cpp:129)
+        const Pack& K0 = packs[P.pack[p]];
+        if (p == K0.p_begin) P.maxdisp[P.pack[p]] = 0ULL;
+        if (K0.ctl_rscale != 1.0 && !(P.flags[p] & SKIP_MACRO_SUM) && is_rattler(P, p)) {
+            const double sc = K0.ctl_rscale;
+            P.vx[p] *= sc; P.vy[p] *= sc; P.w[p] *= sc;
+        }
+    }
     if (P.kind[p] == KIND_ELLIPSE) {
         // src/EllipticParticle.cpp:53-73 (P1: k1, T, I + k1k1 eta) and
         // :211-216 (T^-1, A1); P2's I - k2k2 e2^2 is formed per pair from k
@@ -479,7 +512,7 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
         const Pack K = packs[P.pack[i]];
         P.dsx[i] = 0.0;
         P.dsy[i] = 0.0;
-        if (i == K.p_begin) packs[P.pack[i]].shift_acc = 0.0;
+        if (i == K.p_begin) { packs[P.pack[i]].shift_acc = 0.0; packs[P.pack[i]].sigma = 1.0; }
         const bool skip_i = K.world == WORLD_PERIODIC && (P.flags[i] & SKIP_CONTACT_DETECTION);
         if (!skip_i) {
             const double xi = P.x[i], yi = P.y[i], ri = P.r[i] + K.skin;
@@ -1294,6 +1327,9 @@ k_gather(Parts P, Table t, uint32_t* err) {
         ++nc;
     }
     P.fx[p] = fx; P.fy[p] = fy; P.tq[p] = tq; P.ncont[p] = nc;
+    // setAllParticlesAsRattler, then setAsNonRattler for both partners of
+    // every contact (src/BaseWorld.cpp:25, 44-45)
+    P.rat[p] = nc == 0 ? 1 : 0;
 }
 
 // takeTimeIntegral (src/BaseWorld.cpp:59-77, src/BaseParticle.cpp:107-118,
@@ -1375,7 +1411,8 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
                 // path-length bound of the sub's motion: centre + rotation
                 const double hop = sqrt(vx * vx + vy * vy) * dt + fabs(w) * dt * sqrt(bx * bx + by * by);
                 const double d = (P.dsx[k] += hop);
-                if (2.0 * d + K.shift_acc + fabs(K.shift_vel * K.dt) >= K.skin && *rebuild_next == 0)
+                if (K.ctl_mode != 0) atomicMax(P.maxdisp + pk, (unsigned long long)__double_as_longlong(2.0 * d));
+                else if (2.0 * d + K.shift_acc + fabs(K.shift_vel * K.dt) >= K.skin && *rebuild_next == 0)
                     atomicOr(rebuild_next, 1);
                 P.x[k] = x + bx;
                 P.y[k] = y + by;
@@ -1388,8 +1425,14 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
     // particles together travelled it, plus the LE shift change
     const double dshift = K.world == WORLD_LE ? fabs(K.rate * K.ylen * K.dt) : 0.0;
     const double ddx = P.dsx[p], ddy = P.dsy[p];
-    if (2.0 * sqrt(ddx * ddx + ddy * ddy) + K.shift_acc + dshift >= K.skin && *rebuild_next == 0)
+    const double disp2 = 2.0 * sqrt(ddx * ddx + ddy * ddy);
+    if (K.ctl_mode != 0) {
+        // the boundary update of this step still moves every particle:
+        // k_control decides once it knows the scaling
+        atomicMax(P.maxdisp + pk, (unsigned long long)__double_as_longlong(disp2));
+    } else if (disp2 + K.shift_acc + dshift >= K.skin && *rebuild_next == 0) {
         atomicOr(rebuild_next, 1);
+    }
     if (p == K.p_begin && K.world == WORLD_LE) {
         Pack& W = packs[pk];
         W.shift_vel = K.rate * K.ylen;
@@ -1431,6 +1474,166 @@ k_stress(Parts P, Table t, double* out) {
     }
     out[4 * (size_t)p + 0] = sxx; out[4 * (size_t)p + 1] = sxy;
     out[4 * (size_t)p + 2] = syx; out[4 * (size_t)p + 3] = syy;
+}
+
+// Sum of the per-particle stress contributions of one packing by one
+// workgroup, in a fixed order (strided partial sums, then a tree), divided by
+// the box area: BaseWorld::updateTotalStress (src/BaseWorld.cpp:158-176).
+__device__ void block_stress(const Pack& K, const double* contrib, double* out4) {
+    __shared__ double red[4][kBlock];
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    for (int p = K.p_begin + (int)threadIdx.x; p < K.p_end; p += kBlock) {
+        a += contrib[4 * (size_t)p]; b += contrib[4 * (size_t)p + 1];
+        c += contrib[4 * (size_t)p + 2]; d += contrib[4 * (size_t)p + 3];
+    }
+    red[0][threadIdx.x] = a; red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c; red[3][threadIdx.x] = d;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double area = K.xlen * K.ylen;
+        for (int k = 0; k < 4; ++k) out4[k] = red[k][0] / area;
+    }
+}
+
+// One workgroup per packing: total stress of the packing (out[4k..4k+3]).
+__global__ void __launch_bounds__(kBlock)
+k_stress_sum(const Pack* __restrict__ packs, const double* contrib, double* out) {
+    block_stress(packs[blockIdx.x], contrib, out + 4 * (size_t)blockIdx.x);
+}
+
+// Grid sizes after a change of the box: the reference grid as
+// findPossibleContacts derives it every step (src/PeriodicBoundaryWorld.cpp:15-22);
+// the list grid keeps its cells at least 2 r_max + skin and fewer than the
+// bins allocated.  Returns true when the list grid changed (rebuild).
+__device__ bool regrid(Pack& K, uint32_t* err) {
+    K.nx = (int)floor(K.xlen / K.cell);
+    K.ny = (int)floor(K.ylen / K.cell);
+    if (K.nx < 3 || K.ny < 3) { raise_err(err, ERR_GRID); K.nx = max(K.nx, 3); K.ny = max(K.ny, 3); }
+    K.lenx = K.xlen / K.nx;
+    K.leny = K.ylen / K.ny;
+    const int tx = max(3, min(K.vnx_alloc, (int)floor(K.xlen / K.vcell)));
+    const int ty = max(3, min(K.vny_alloc, (int)floor(K.ylen / K.vcell)));
+    bool changed = false;
+    if (tx < K.vnx) { K.vnx = tx; changed = true; }
+    if (ty < K.vny) { K.vny = ty; changed = true; }
+    K.vlenx = K.xlen / K.vnx;
+    K.vleny = K.ylen / K.vny;
+    if (K.vlenx < K.vcell || K.vleny < K.vcell) {
+        const double sk = fmin(K.vlenx, K.vleny) - 2.0 * K.rmax;
+        if (sk < 0.0) raise_err(err, ERR_GRID);
+        K.skin = fmax(0.0, sk);
+        changed = true;
+    }
+    return changed;
+}
+
+// The boundary updates of PeriodicControlWorld (src/PeriodicControlWorld.cpp:124-237;
+// MAX_SCALE_RATE :7), once per step after updateTotalStress, in the order of
+// the drivers' compression loop (mainIsoElpsPack_WCCM.cpp:135-137).  One
+// workgroup per packing; thread 0 updates the box and its grids and sets the
+// scaling that k_ctl_particles applies (scaleAllParticlePosition,
+// src/BaseWorld.cpp:119-124).  The pair list stays valid while
+// 2 max|displacement| + (1 - sigma) 2 r_max < sigma skin, sigma the product of
+// the shrink factors since the rebuild: a pair that was at least
+// r_i + r_j + skin apart is then still farther apart than r_i + r_j.
+constexpr double kMaxScaleRate = 100000000.0;
+
+__global__ void __launch_bounds__(kBlock)
+k_control(Parts P, Pack* packs, const double* contrib, int* rebuild_next, uint32_t* err) {
+    Pack& KW = packs[blockIdx.x];
+    const int mode = KW.ctl_mode;
+    if (mode == 0) {
+        if (threadIdx.x == 0) { KW.scale_x = 1.0; KW.scale_y = 1.0; }
+        return;
+    }
+    __shared__ double st[4];
+    block_stress(KW, contrib, st);
+    if (threadIdx.x != 0) return;
+    Pack K = KW;
+    for (int k = 0; k < 4; ++k) K.tstress[k] = st[k];
+    const double dt = K.dt;
+    double sx = 1.0, sy = 1.0;
+    if (mode == XDEM_CONTROL_STRAIN) {                       // :124-135
+        const double scaleX = 1.0 + K.ctl_rx * K.xlen * dt;
+        const double scaleY = 1.0 + K.ctl_ry * K.ylen * dt;
+        K.xmin *= scaleX; K.xmax *= scaleX;
+        K.ymin *= scaleY; K.ymax *= scaleY;
+        K.xlen = K.xmax - K.xmin; K.ylen = K.ymax - K.ymin;
+        sx = scaleX; sy = scaleY;
+    } else if (mode == XDEM_CONTROL_STRESS_UNIFORM) {        // :137-151
+        const double hydro = (K.tstress[0] + K.tstress[3]) / 2.0;
+        const double deltaVel = (K.ctl_stress - hydro) * K.xlen * dt;
+        K.velx += deltaVel;
+        K.vely = K.velx;
+        const double scale = (K.xmin + deltaVel * dt) / K.xmin;
+        K.xmin += deltaVel * dt; K.xmax = -K.xmin; K.xlen = K.xmax - K.xmin;
+        K.ymin += deltaVel * dt; K.ymax = -K.ymin; K.ylen = K.ymax - K.ymin;
+        sx = scale; sy = scale;
+    } else if (mode == XDEM_CONTROL_STRESS_XY) {             // :153-193, X then Y
+        {
+            const double xminPre = K.xmin;
+            const double stressX = K.tstress[0];
+            if (fabs(stressX) < K.ctl_stress) K.velx *= (1.0 - dt * 1.0);
+            const double deltaVel = (K.ctl_stress - stressX) * K.xlen / K.bmass * dt;
+            K.velx = deltaVel;
+            if (fabs(K.velx) / K.xlen > kMaxScaleRate * dt) K.velx = kMaxScaleRate * dt * K.xlen;
+            K.xmin += K.velx * dt; K.xmax = -K.xmin; K.xlen = K.xmax - K.xmin;
+            sx = K.xmin / xminPre;
+        }
+        {
+            const double yminPre = K.ymin;
+            const double stressY = K.tstress[3];
+            if (fabs(stressY) < K.ctl_stress) K.vely *= (1.0 - dt * 1.0);
+            const double deltaVel = (K.ctl_stress - stressY) * K.ylen / K.bmass * dt;
+            K.vely = deltaVel;
+            if (fabs(K.vely) / K.ylen > kMaxScaleRate * dt) K.vely = kMaxScaleRate * dt * K.ylen;
+            K.ymin += K.vely * dt; K.ymax = -K.ymin; K.ylen = K.ymax - K.ymin;
+            sy = K.ymin / yminPre;
+        }
+    } else if (mode == XDEM_CONTROL_STRESS_LIMITED) {        // :213-237
+        const double xminPre = K.xmin;
+        const double hydro = (K.tstress[0] + K.tstress[3]) / 2.0;
+        const double deltaVel = (K.ctl_stress - hydro) * K.xlen * dt;
+        K.velx += deltaVel;
+        if ((fabs(K.velx) / (K.xlen / 2.0)) > K.max_rate)
+            K.velx = fabs(K.velx) / K.velx * (K.xlen / 2.0) * K.max_rate;
+        K.vely = K.velx;
+        K.xmin += K.velx * dt; K.xmax = -K.xmin; K.xlen = K.xmax - K.xmin;
+        K.ymin += K.vely * dt; K.ymax = -K.ymin; K.ylen = K.ymax - K.ymin;
+        const double scale = K.xmin / xminPre;
+        sx = scale; sy = scale;
+    }
+    K.scale_x = sx; K.scale_y = sy;
+    K.hx = K.xlen / 2.0; K.hy = K.ylen / 2.0;
+    const bool changed = regrid(K, err);
+    const double shrink = fmin(1.0, fmin(sx, sy));
+    K.sigma = K.sigma * (shrink > 0.0 ? shrink : 0.0);
+    const double md = __longlong_as_double((long long)P.maxdisp[blockIdx.x]);
+    if (changed || !(md + K.shift_acc + (1.0 - K.sigma) * 2.0 * K.rmax < K.sigma * K.skin))
+        atomicOr(rebuild_next, 1);
+    KW = K;
+}
+
+// After the boundary update: resetVelocityOfRattlers (src/BaseWorld.cpp:79-92;
+// PeriodicControlWorld::takeTimeStep, src/PeriodicControlWorld.cpp:114-122),
+// then the position scaling of the boundary update.
+__global__ void __launch_bounds__(kBlock)
+k_ctl_particles(Parts P, const Pack* __restrict__ packs, const uint32_t* err) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= P.n || *err) return;
+    const Pack& K = packs[P.pack[p]];
+    if (K.ctl_reset && !(P.flags[p] & SKIP_MACRO_SUM) && is_rattler(P, p)) {
+        P.vx[p] = 0.0; P.vy[p] = 0.0; P.w[p] = 0.0;
+    }
+    if (K.ctl_mode != 0) {
+        P.x[p] *= K.scale_x;
+        P.y[p] *= K.scale_y;
+    }
 }
 
 // Fixed ellipse quantities of src/EllipticParticle.cpp:49-61, 72: b = a/AR,
@@ -1534,6 +1737,9 @@ struct xdem_engine {
     // root log
     RootLog log{};
     bool logging = false;
+    // boundary control / rattler treatment set on some packing
+    bool control_active = false;
+    double* box_stress_d = nullptr;   // 4 per packing
 };
 
 namespace {
@@ -1578,6 +1784,7 @@ int map_error_bits(xdem_engine* E, uint32_t bits) {
     if (bits & ERR_INCOMING_OVERFLOW) { m += " more than 32 contacts held by partners of one particle;"; code = XDEM_E_CAPACITY; }
     if (bits & ERR_NONFINITE) { m += " a particle position became non-finite (coincident centres give 0/0 normals);"; code = XDEM_E_NONFINITE; }
     if (bits & ERR_ROW_OVERFLOW) { m += " more than 512 bounding-circle pairs in one cell;"; code = XDEM_E_CAPACITY; }
+    if (bits & ERR_GRID) { m += " the boundary control shrank a cell grid below 3x3;"; code = XDEM_E_GRID; }
     return fail(E, code, m);
 }
 
@@ -1785,12 +1992,16 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         K.skin = c.skin > 0 ? c.skin : 0.2 * rmax;
         K.shift_acc = 0.0;
         const double vcell = 2.0 * rmax + K.skin;
+        K.cell = cell; K.rmax = rmax; K.sigma = 1.0;
+        K.ctl_rscale = 1.0; K.scale_x = 1.0; K.scale_y = 1.0;
         K.vnx = std::max(3, (int)std::floor(K.xlen / vcell));
         K.vny = std::max(3, (int)std::floor(K.ylen / vcell));
         K.vlenx = K.xlen / K.vnx;
         K.vleny = K.ylen / K.vny;
         if (K.vlenx < vcell || K.vleny < vcell) K.skin = std::max(0.0, std::min(K.vlenx, K.vleny) - 2.0 * rmax);
         K.hx = K.xlen / 2.0; K.hy = K.ylen / 2.0;
+        K.vnx_alloc = K.vnx; K.vny_alloc = K.vny;
+        K.vcell = 2.0 * rmax + K.skin;      // after the skin fits the grid
         K.bin_base = bins_total;
         bins_total += (K.vny + 2) * K.vnx;
         if (amin < 1e300) max_ratio = std::max(max_ratio, K.vlenx * K.vleny / amin);
@@ -1901,6 +2112,9 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     HIPCHK(hipMemset(P.in_cnt, 0, ib));
     HIPCHK(hipMemset(P.dsx, 0, db));
     HIPCHK(hipMemset(P.dsy, 0, db));
+    if ((rc = dalloc(E, &P.rat, n)) || (rc = dalloc(E, &P.maxdisp, E->npacks))) return rc;
+    HIPCHK(hipMemset(P.rat, 0, (size_t)n));
+    HIPCHK(hipMemset(P.maxdisp, 0, sizeof(unsigned long long) * E->npacks));
     // bins
     E->B.cap = bin_cap;
     E->B.nbins = bins_total;
@@ -1941,7 +2155,8 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         return rc;
     if ((rc = dalloc(E, &E->packs_d, E->npacks)) || (rc = dalloc(E, &E->err_d, 1)) ||
         (rc = dalloc(E, &E->cnt_d, kSeg)) || (rc = dalloc(E, &E->stress_d, (size_t)4 * n)) ||
-        (rc = dalloc(E, &E->rebuild_d, 2)))
+        (rc = dalloc(E, &E->rebuild_d, 2)) ||
+        (rc = dalloc(E, &E->box_stress_d, (size_t)4 * E->npacks)))
         return rc;
     HIPCHK(hipMemcpy(E->packs_d, E->packs_h.data(), sizeof(Pack) * E->npacks, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(E->err_d, 0, sizeof(uint32_t)));
@@ -2069,6 +2284,15 @@ int launch_step_t(xdem_engine* E) {
     const int gi = std::max(gn, grid_of(E->B.nbins) / 4 + 1);
     hipLaunchKernelGGL(k_integrate, dim3(gi), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
                        E->B, old_t.counter, rb, rb_next, E->err_d);
+    if (E->control_active) {
+        // updateTotalStress, boundary update, rattler reset, position scaling
+        hipLaunchKernelGGL(k_stress, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t,
+                           E->stress_d);
+        hipLaunchKernelGGL(k_control, dim3(E->npacks), dim3(kBlock), 0, E->stream, E->P,
+                           E->packs_d, E->stress_d, rb_next, E->err_d);
+        hipLaunchKernelGGL(k_ctl_particles, dim3(gn), dim3(kBlock), 0, E->stream, E->P,
+                           E->packs_d, E->err_d);
+    }
     HIPCHK(hipGetLastError());
     E->cur ^= 1;
     E->steps++;
@@ -2220,23 +2444,16 @@ int xdem_get_stress(xdem_engine* E, double* out) {
     int rc = xdem_sync(E);
     if (rc) return rc;
     if (!out) return fail(E, XDEM_E_INVALID, "out is NULL");
+    // per-particle contributions, then one workgroup per packing sums them
+    // in a fixed order over the current box
     hipLaunchKernelGGL(k_stress, dim3(grid_of(E->n)), dim3(kBlock), 0, E->stream, E->P,
                        E->T[E->cur], E->stress_d);
+    hipLaunchKernelGGL(k_stress_sum, dim3(E->npacks), dim3(kBlock), 0, E->stream, E->packs_d,
+                       E->stress_d, E->box_stress_d);
     HIPCHK(hipGetLastError());
-    std::vector<double> s((size_t)4 * E->n);
-    HIPCHK(hipMemcpyAsync(s.data(), E->stress_d, sizeof(double) * s.size(), hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipMemcpyAsync(out, E->box_stress_d, sizeof(double) * 4 * E->npacks,
+                          hipMemcpyDeviceToHost, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
-    for (int k = 0; k < E->npacks; ++k) {
-        double a = 0, b = 0, c = 0, d = 0;
-        for (int p = E->pack_start[k]; p < E->pack_start[k + 1]; ++p) {
-            const size_t q = 4 * (size_t)E->inv[p];
-            a += s[q]; b += s[q + 1]; c += s[q + 2]; d += s[q + 3];
-        }
-        const Pack& K = E->packs_h[k];
-        const double area = K.xlen * K.ylen;
-        out[4 * k] = a / area; out[4 * k + 1] = b / area;
-        out[4 * k + 2] = c / area; out[4 * k + 3] = d / area;
-    }
     return XDEM_OK;
 }
 
@@ -2247,6 +2464,77 @@ int xdem_get_boundary(xdem_engine* E, double* out) {
     HIPCHK(hipMemcpy(h.data(), E->packs_d, sizeof(Pack) * E->npacks, hipMemcpyDeviceToHost));
     for (int k = 0; k < E->npacks; ++k) {
         out[3 * k] = h[k].shift; out[3 * k + 1] = h[k].shift_vel; out[3 * k + 2] = h[k].strain;
+    }
+    return XDEM_OK;
+}
+
+int xdem_set_control(xdem_engine* E, int packing, const xdem_control* c) {
+    if (!E || !E->have_particles) return fail(E, XDEM_E_INVALID, "set particles first");
+    if (!c || packing < 0 || packing >= E->npacks) return fail(E, XDEM_E_INVALID, "bad packing");
+    if (c->mode < XDEM_CONTROL_NONE || c->mode > XDEM_CONTROL_STRESS_LIMITED)
+        return fail(E, XDEM_E_INVALID, "unknown control mode");
+    if (!(c->rattler_scale == c->rattler_scale))
+        return fail(E, XDEM_E_INVALID, "rattler_scale is NaN");
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    std::vector<Pack> h(E->npacks);
+    HIPCHK(hipMemcpy(h.data(), E->packs_d, sizeof(Pack) * E->npacks, hipMemcpyDeviceToHost));
+    Pack& K = h[packing];
+    if (c->mode != XDEM_CONTROL_NONE && K.world != XDEM_WORLD_PERIODIC)
+        return fail(E, XDEM_E_INVALID, "boundary control needs a periodic packing");
+    K.ctl_mode = c->mode;
+    K.ctl_reset = c->reset_rattlers ? 1 : 0;
+    K.ctl_rscale = c->rattler_scale;
+    K.ctl_stress = c->stress;
+    K.ctl_rx = c->strain_rate_x;
+    K.ctl_ry = c->strain_rate_y;
+    K.max_rate = c->max_strain_rate;
+    if (c->boundary_mass > 0) {
+        K.bmass = c->boundary_mass;
+    } else {
+        // setVirtualBoundaryMass(): sum over particles in ID order
+        // (src/PeriodicControlWorld.cpp:58-66)
+        std::vector<double> m(E->n);
+        std::vector<int8_t> f(E->n);
+        HIPCHK(hipMemcpy(m.data(), E->P.m, sizeof(double) * E->n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(f.data(), E->P.flags, (size_t)E->n, hipMemcpyDeviceToHost));
+        double acc = 0.0;
+        const int b = E->pack_start[packing], e = E->pack_start[packing + 1];
+        for (int p = b; p < e; ++p) {
+            const int q = E->inv[p];
+            if (f[q] & SKIP_MACRO_SUM) continue;
+            acc += m[q];
+        }
+        K.bmass = acc / (double(e - b) * 10.0);
+    }
+    K.scale_x = 1.0; K.scale_y = 1.0;
+    HIPCHK(hipMemcpy(E->packs_d, h.data(), sizeof(Pack) * E->npacks, hipMemcpyHostToDevice));
+    bool active = false;
+    for (const Pack& q : h) active |= q.ctl_mode != 0 || q.ctl_reset || q.ctl_rscale != 1.0;
+    if (active != E->control_active) {
+        // the captured step sequence changes: capture again
+        for (int k = 0; k < 2; ++k) {
+            if (E->graph_exec[k]) hipGraphExecDestroy(E->graph_exec[k]);
+            if (E->graph[k]) hipGraphDestroy(E->graph[k]);
+            E->graph_exec[k] = nullptr;
+            E->graph[k] = nullptr;
+        }
+        E->control_active = active;
+    }
+    return XDEM_OK;
+}
+
+int xdem_get_box(xdem_engine* E, double* out) {
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    if (!out) return fail(E, XDEM_E_INVALID, "out is NULL");
+    std::vector<Pack> h(E->npacks);
+    HIPCHK(hipMemcpy(h.data(), E->packs_d, sizeof(Pack) * E->npacks, hipMemcpyDeviceToHost));
+    for (int k = 0; k < E->npacks; ++k) {
+        double* o = out + 10 * (size_t)k;
+        o[0] = h[k].xmin; o[1] = h[k].xmax; o[2] = h[k].ymin; o[3] = h[k].ymax;
+        o[4] = h[k].velx; o[5] = h[k].vely;
+        for (int q = 0; q < 4; ++q) o[6 + q] = h[k].tstress[q];
     }
     return XDEM_OK;
 }

@@ -27,6 +27,7 @@ SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles
            "xdem_set_contacts", "xdem_step", "xdem_sync", "xdem_get_particles",
            "xdem_get_forces", "xdem_get_contacts", "xdem_get_stress", "xdem_get_boundary",
            "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
+           "xdem_set_control", "xdem_get_box",
            "xdem_set_mlp", "xdem_get_mlp", "xdem_mlp_init", "xdem_log_roots",
            "xdem_log_count", "xdem_log_fetch", "xdem_mlp_train", "xdem_mlp_grad",
            "xdem_mlp_predict")
@@ -84,6 +85,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_set_timing.argtypes = [vp, C.c_int]
     L.xdem_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.xdem_get_traffic.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.xdem_set_control.argtypes = [vp, C.c_int, C.POINTER(XdemControl)]
+    L.xdem_get_box.argtypes = [vp, _D]
     L.xdem_set_mlp.argtypes = [vp, _F]
     L.xdem_get_mlp.argtypes = [vp, _F]
     L.xdem_mlp_init.argtypes = [vp, C.c_uint]
@@ -95,6 +98,18 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_mlp_predict.argtypes = [vp, C.c_int, _F, _F]
     _lib = L
     return L
+
+
+class XdemControl(C.Structure):
+    """xdem_control of include/xdem.h (PeriodicControlWorld boundary control)."""
+    _fields_ = [("mode", C.c_int), ("reset_rattlers", C.c_int), ("rattler_scale", C.c_double),
+                ("stress", C.c_double), ("strain_rate_x", C.c_double),
+                ("strain_rate_y", C.c_double), ("boundary_mass", C.c_double),
+                ("max_strain_rate", C.c_double)]
+
+
+CONTROL_NONE, CONTROL_STRAIN, CONTROL_STRESS_UNIFORM, CONTROL_STRESS_XY, \
+    CONTROL_STRESS_LIMITED = 0, 1, 2, 3, 4
 
 
 class XdemError(RuntimeError):
@@ -205,6 +220,23 @@ class Engine:
         out = np.zeros(4 * len(self.states))
         self._check(self.L.xdem_get_stress(self.h, out))
         return out.reshape(-1, 4)
+
+    def set_control(self, mode, packing=0, stress=0.0, rate_x=0.0, rate_y=0.0,
+                    rattler_scale=1.0, reset_rattlers=False, boundary_mass=0.0,
+                    max_strain_rate=0.0):
+        """Boundary control and rattler treatment applied inside every step
+        of `packing` (xdem_set_control)."""
+        c = XdemControl(int(mode), int(bool(reset_rattlers)), float(rattler_scale),
+                        float(stress), float(rate_x), float(rate_y), float(boundary_mass),
+                        float(max_strain_rate))
+        self._check(self.L.xdem_set_control(self.h, int(packing), C.byref(c)))
+
+    def box(self):
+        """Per packing: xmin, xmax, ymin, ymax, boundary velocity x, y and the
+        stress of the last control step; shape (packings, 10)."""
+        out = np.zeros(10 * len(self.states))
+        self._check(self.L.xdem_get_box(self.h, out))
+        return out.reshape(-1, 10)
 
     def boundary(self):
         out = np.zeros(3 * len(self.states))

@@ -58,6 +58,7 @@ def lib():
         L.oracle_bisect.argtypes = [C.c_double] * 6 + [C.POINTER(C.c_double)]
         L.oracle_set_control.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6
         L.oracle_get_box.argtypes = [C.c_void_p, _D]
+        L.oracle_get_observables.argtypes = [C.c_void_p, _D]
         _lib = L
     return _lib
 
@@ -145,6 +146,15 @@ class OracleWorld:
                                              float(rate_x), float(rate_y),
                                              float(boundary_mass), float(max_strain_rate)))
 
+    def observables(self):
+        """countPossibleContacts, countActucalContacts, getKineticEnergy,
+        getKineticEnergyPerParticle, getKineticEnergyPerNonRattlerParticle,
+        getElasticEnergy, getElasticEnergyPerContact, total particle volume,
+        getSolidFraction, non-rattler count (src/BaseWorld.cpp:190-334)."""
+        out = np.zeros(10)
+        lib().oracle_get_observables(self.h, out)
+        return out
+
     def box(self):
         """xmin, xmax, ymin, ymax, boundary velocity x, y, last control stress (4)."""
         out = np.zeros(10)
@@ -212,7 +222,7 @@ def have_reference() -> bool:
 def read_golden(path):
     """Parse a ref_harness `run` dump into a list of per-step dicts."""
     buf = open(path, "rb").read()
-    assert buf[:8] == b"XDEMGD01"
+    assert buf[:8] == b"XDEMGD02"
     off = 8
     steps = []
 
@@ -236,6 +246,10 @@ def read_golden(path):
             d[k] = rec[:, col].copy()
         d["stress"] = take(np.float64, 4)
         d["boundary"] = take(np.float64, 3)
+        # countPossibleContacts, countActucalContacts, getKineticEnergy,
+        # getKineticEnergyPerParticle, getKineticEnergyPerNonRattlerParticle,
+        # getElasticEnergy, getElasticEnergyPerContact, getSolidFraction
+        d["observables"] = take(np.float64, 8)
         steps.append(d)
     return steps
 
@@ -255,9 +269,9 @@ def read_control_golden(path):
         n = int(np.frombuffer(body, np.int32, 2, pos)[1])
         p = pos + 8 + 9 * 8 * n + 4 * n
         nc = int(np.frombuffer(body, np.int32, 1, p)[0])
-        end = p + 4 + 8 * nc + 13 * 8 * nc + 4 * 8 + 3 * 8
+        end = p + 4 + 8 * nc + 13 * 8 * nc + 4 * 8 + 3 * 8 + 8 * 8
         with open(tmp, "wb") as f:
-            f.write(b"XDEMGD01" + body[pos:end])
+            f.write(b"XDEMGD02" + body[pos:end])
         d = read_golden(tmp)[0]
         d["box"] = np.frombuffer(body, np.float64, 10, end).copy()
         steps.append(d)

@@ -5,7 +5,8 @@
 // Modes
 //   ref_harness run   <state.xst> <out.xdg> <nsteps> <dump_every> [threads]
 //       builds the reference world from a state file, runs takeTimeStep and
-//       dumps particle state, forces, contacts, stress and LE scalars.
+//       dumps particle state, forces, contacts, stress, LE scalars and the
+//       world's observables.
 //   ref_harness pairs <pairs.bin> <out.bin>
 //       calls EllipticParticle / CircularParticle::calculateContactGeometry
 //       on a list of poses (src/EllipticParticle.cpp:16-32).
@@ -196,6 +197,7 @@ void build(Built& B, int threads, int chunk, bool control = false) {
         B.pb->setWorldBoundary(c[0], c[1], c[2], c[3]);
     }
     B.world()->set_OMP_NumThreadsAndSizeChunk(threads, chunk);
+    B.world()->updateTotalParticleVolume();
 }
 
 template <class T> void wr(FILE* f, const T* p, size_t n) { if (n) fwrite(p, sizeof(T), n, f); }
@@ -254,6 +256,12 @@ void dump(Built& B, FILE* f, int step) {
     double sc[3] = {0, 0, 0};
     if (B.le) B.le->scalars(sc);
     wr(f, sc, 3);
+    // observables (src/BaseWorld.cpp:251-334)
+    double ob[8] = {(double)W->countPossibleContacts(), (double)W->countActucalContacts(),
+                    W->getKineticEnergy(), W->getKineticEnergyPerParticle(),
+                    W->getKineticEnergyPerNonRattlerParticle(), W->getElasticEnergy(),
+                    W->getElasticEnergyPerContact(), W->getSolidFraction()};
+    wr(f, ob, 8);
 }
 
 int mode_run(int argc, char** argv) {
@@ -265,7 +273,7 @@ int mode_run(int argc, char** argv) {
     build(B, threads, 100);
     FILE* f = fopen(argv[3], "wb");
     if (!f) return 3;
-    fwrite("XDEMGD01", 1, 8, f);
+    fwrite("XDEMGD02", 1, 8, f);
     for (int s = 1; s <= nsteps; ++s) {
         B.world()->takeTimeStep();
         if (every > 0 && (s % every == 0 || s == nsteps)) dump(B, f, s);

@@ -108,6 +108,7 @@ struct World {
     double tstress[4] = {0, 0, 0, 0};
     std::vector<char> rattler;
     double volume = 0.0;                     // _totalParticleVolume
+    long long possible = 0;                  // countPossibleContacts of the last step
 };
 
 // ---------------------------------------------------------------------------
@@ -649,7 +650,8 @@ static int step_once(World& W) {
     std::vector<int> errj(n, -1);
     const double hx = W.xlen / 2.0, hy = W.ylen / 2.0;
     long long cand_total = 0, newton_total = 0, coarse_total = 0, rescue_total = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : cand_total, newton_total, coarse_total, rescue_total)
+    long long possible_total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : cand_total, newton_total, coarse_total, rescue_total, possible_total)
     for (int i = 0; i < n; ++i) {
         Counters K;
         if (!le && (W.flags[i] & SKIP_CONTACT_DETECTION)) continue;
@@ -679,6 +681,8 @@ static int step_once(World& W) {
             continue;
         }
         cand_total += lists[0].size() + lists[1].size() + lists[2].size();
+        // countPossibleContacts counts the normal lists only (src/BaseWorld.cpp:251-260)
+        possible_total += (long long)lists[0].size();
         for (int j : lists[0]) {
             double ox = 0.0, oy = 0.0;
             if (W.x[j] - W.x[i] > hx) ox = -W.xlen;
@@ -709,6 +713,7 @@ static int step_once(World& W) {
         rescue_total += K.rescues;
     }
     W.candidates += cand_total;
+    W.possible = possible_total;
     W.newton_iters += newton_total;
     W.coarse += coarse_total;
     W.rescues += rescue_total;
@@ -997,6 +1002,55 @@ int oracle_get_box(void* h, double* out) {
     out[0] = W.xmin; out[1] = W.xmax; out[2] = W.ymin; out[3] = W.ymax;
     out[4] = W.velx; out[5] = W.vely;
     for (int k = 0; k < 4; ++k) out[6 + k] = W.tstress[k];
+    return XDEM_OK;
+}
+
+// BaseWorld observables (src/BaseWorld.cpp:190-199, 251-334):
+// out = countPossibleContacts, countActucalContacts, getKineticEnergy,
+// getKineticEnergyPerParticle, getKineticEnergyPerNonRattlerParticle,
+// getElasticEnergy, getElasticEnergyPerContact, total particle volume,
+// getSolidFraction, non-rattler count.  A clump's volume is its mass
+// (density 1; GluedParticle::calculateParticleProperties sets mass =
+// density * volume, src/GluedParticle.cpp:40-56).
+int oracle_get_observables(void* h, double* out) {
+    World& W = *(World*)h;
+    long long actual = 0;
+    double el = 0.0;
+    for (int i = 0; i < W.n; ++i) {
+        for (auto& kv : W.hist[i]) {
+            const Contact& c = kv.second;
+            const double ox = c.fnx / W.kn, oy = c.fny / W.kn;
+            el += ox * ox + oy * oy;
+            ++actual;
+        }
+    }
+    double ke = 0.0, ken = 0.0;
+    int nonr = 0;
+    for (int i = 0; i < W.n; ++i) {
+        const double k = 0.5 * (W.m[i] * (W.vx[i] * W.vx[i] + W.vy[i] * W.vy[i]) +
+                                W.in[i] * W.w[i] * W.w[i]);
+        ke += k;
+        if (is_rattler(W, i)) continue;
+        ken += k;
+        ++nonr;
+    }
+    double vol = 0.0;
+    for (int i = 0; i < W.n; ++i) {
+        if (W.flags[i] & SKIP_MACRO_SUM) continue;
+        if (W.kind[i] == KIND_ELLIPSE) vol += M_PI * W.r[i] * W.r[i] / W.ar[i];
+        else if (W.kind[i] == KIND_CIRCLE) vol += M_PI * W.r[i] * W.r[i];
+        else vol += W.m[i];
+    }
+    out[0] = (double)W.possible;
+    out[1] = (double)actual;
+    out[2] = ke;
+    out[3] = ke / W.n;
+    out[4] = nonr == 0 ? 0.0 : ken / nonr;
+    out[5] = el * W.kn * 0.5;
+    out[6] = actual == 0 ? 0.0 : el * W.kn * 0.5 / actual;
+    out[7] = vol;
+    out[8] = vol / (W.xlen * W.ylen);
+    out[9] = (double)nonr;
     return XDEM_OK;
 }
 

@@ -97,3 +97,51 @@ def test_control_against_reference(name):
             assert float(np.max(np.abs(p[k] - g[k]))) <= 1e-11 * scale, f"{name}: {k} step {st}"
         if mode != 1:
             assert np.max(np.abs(box[4:6])) > 0, "the boundary should move"
+
+
+OBS_NAMES = ("possible_contacts", "actual_contacts", "kinetic_energy", "kinetic_per_particle",
+             "kinetic_per_nonrattler", "elastic_energy", "elastic_per_contact", "solid_fraction")
+ORACLE_OBS = (0, 1, 2, 3, 4, 5, 6, 8)    # oracle observables() entries of the eight
+
+
+def check_observables(o, g, where, clumps):
+    """The world observables of the oracle against the reference's:
+    contact counts exactly, energies within 1e-12 relative (summation order),
+    solid fraction within 1e-14 (clump volumes differ: the harness builds
+    clumps without calculateParticleProperties, so their volume is 0)."""
+    for name, k, ref in zip(OBS_NAMES, ORACLE_OBS, g):
+        v = o[k]
+        if name.endswith("contacts"):
+            assert v == ref, f"{where}: {name} {v} != {ref}"
+        elif name == "solid_fraction":
+            if not clumps:
+                assert abs(v - ref) <= 1e-14 * abs(ref), f"{where}: {name}"
+        else:
+            assert abs(v - ref) <= 1e-12 * max(abs(ref), 1e-300), f"{where}: {name} {v} vs {ref}"
+
+
+@pytest.mark.parametrize("name", WORLD_CASES)
+def test_observables_against_reference(name):
+    s, steps = load_world(name)
+    w = po.OracleWorld(s)
+    done = 0
+    for st in DUMP_STEPS:
+        w.step(st - done)
+        done = st
+        check_observables(w.observables(), steps[st]["observables"], f"{name} step {st}",
+                          "clumps" in name)
+    assert steps[DUMP_STEPS[-1]]["observables"][0] > 0
+
+
+@pytest.mark.parametrize("name", __import__("tests.golden_io", fromlist=["x"]).CONTROL_CASES)
+def test_control_observables_against_reference(name):
+    from tests.golden_io import CONTROL_DUMP, load_control
+    s, mode, kw, steps = load_control(name)
+    w = po.OracleWorld(s)
+    w.set_control(mode, **kw)
+    done = 0
+    for st in CONTROL_DUMP:
+        w.step(st - done)
+        done = st
+        check_observables(w.observables(), steps[st]["observables"], f"{name} step {st}",
+                          name == "clumps")

@@ -219,6 +219,17 @@ int xdem_set_control(xdem_engine* e, int packing, const xdem_control* c);
  * control step followed (src/PeriodicControlWorld.h:8-11, src/BaseWorld.h). */
 int xdem_get_box(xdem_engine* e, double* out);
 
+/* World observables of each packing (src/BaseWorld.cpp:190-199, 251-334),
+ * as the drivers print them (src/PeriodicBoundaryWorld.cpp:293-307):
+ * out[10k..10k+9] = countPossibleContacts, countActucalContacts,
+ * getKineticEnergy, getKineticEnergyPerParticle,
+ * getKineticEnergyPerNonRattlerParticle, getElasticEnergy,
+ * getElasticEnergyPerContact, total particle volume (updateTotalParticleVolume;
+ * a clump counts its mass, i.e. density 1), getSolidFraction, number of
+ * non-rattler particles.  Energies use the current state; contact counts,
+ * elastic energy and rattlers refer to the last step.  Computed on the device. */
+int xdem_get_observables(xdem_engine* e, double* out);
+
 /* ---- Learned quartic-root guess (XDEM_PREC_NN) --------------------------
  * No counterpart in the reference, whose Newton iteration always starts at
  * s = sqrt(delta + 1) (NewtonRaphsonForQuartic, src/EllipticParticle.cpp:163,

@@ -1,0 +1,62 @@
+"""GPU parity of the world observables (xdem_get_observables) against the
+CPU oracle, whose observables the golden fixtures pin to the reference
+(tests/test_oracle_golden.py::test_observables_against_reference).
+
+Bars: contact counts (possible, actual) and the non-rattler count exactly;
+energies within 1e-10 relative (device sums run in another order); volume and
+solid fraction within 1e-13 relative."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.golden_io import load_control, load_world
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(oe, oo, where):
+    for k in (0, 1, 9):
+        assert oe[k] == oo[k], f"{where}: observable {k}: {oe[k]} != {oo[k]}"
+    for k in (2, 3, 4, 5, 6):
+        assert abs(oe[k] - oo[k]) <= 1e-10 * max(abs(oo[k]), 1e-300), f"{where}: observable {k}"
+    for k in (7, 8):
+        assert abs(oe[k] - oo[k]) <= 1e-13 * abs(oo[k]), f"{where}: observable {k}"
+
+
+@pytest.mark.parametrize("name", ["le_poly", "pb_ellipse", "pb_circle", "pb_clumps",
+                                  "le_poly_hist"])
+def test_observables_match_oracle(name):
+    from xdem_amd.engine import Engine
+    s, _ = load_world(name)
+    eng, orc = Engine(s), po.OracleWorld(s)
+    for nsteps in (1, 3):
+        eng.step(nsteps)
+        orc.step(nsteps)
+        _check(eng.observables()[0], orc.observables(), f"{name} after {nsteps}")
+    assert eng.observables()[0][0] > 0
+
+
+def test_observables_under_control():
+    """After boundary updates the counts refer to the grid of the last step."""
+    from xdem_amd.engine import Engine
+    s, mode, kw, _ = load_control("stress_uniform")
+    eng, orc = Engine(s), po.OracleWorld(s)
+    eng.set_control(mode, **kw)
+    orc.set_control(mode, **kw)
+    eng.step(5)
+    orc.step(5)
+    _check(eng.observables()[0], orc.observables(), "stress_uniform after 5")
+
+
+def test_observables_per_packing():
+    """Several packings in one engine: each packing's observables as alone."""
+    from xdem_amd.engine import Engine
+    s1, _ = load_world("le_poly")
+    s2, _ = load_world("pb_circle")
+    ens = Engine([s1, s2])
+    ens.step(2)
+    obs = ens.observables()
+    for k, s in enumerate((s1, s2)):
+        o = po.OracleWorld(s)
+        o.step(2)
+        _check(obs[k], o.observables(), f"packing {k}")

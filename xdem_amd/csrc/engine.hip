@@ -71,6 +71,7 @@ struct Pack {
     double tstress[4];            // stress of the last control step
     double scale_x, scale_y;      // this step's position scaling
     double cell, vcell, rmax;     // reference-grid cell; smallest list-grid cell
+    int pnx, pny;                 // reference grid of the last step (the cells in P.cell)
     double sigma;                 // product of the shrink factors since the rebuild
 };
 
@@ -96,6 +97,7 @@ struct alignas(32) FRec {
 struct Parts {
     double *x, *y, *th, *vx, *vy, *w;
     const double *r, *ar, *m, *im, *iin;
+    const double* inr;            // moment of inertia (BaseParticle::_inertia)
     double *cbx, *cby;
     const int* up;
     const int* eid;               // external (caller) index of each particle
@@ -1609,6 +1611,7 @@ k_control(Parts P, Pack* packs, const double* contrib, int* rebuild_next, uint32
         sx = scale; sy = scale;
     }
     K.scale_x = sx; K.scale_y = sy;
+    K.pnx = K.nx; K.pny = K.ny;      // the grid this step's cells refer to
     K.hx = K.xlen / 2.0; K.hy = K.ylen / 2.0;
     const bool changed = regrid(K, err);
     const double shrink = fmin(1.0, fmin(sx, sy));
@@ -1634,6 +1637,155 @@ k_ctl_particles(Parts P, const Pack* __restrict__ packs, const uint32_t* err) {
         P.x[p] *= K.scale_x;
         P.y[p] *= K.scale_y;
     }
+}
+
+// ---------------------------------------------------------------------------
+// World observables on demand (src/BaseWorld.cpp:190-199, 251-334).
+// countPossibleContacts counts the candidate lists of the last
+// findPossibleContacts; the reference counts its normal lists only
+// (_possibleContacts, src/BaseWorld.cpp:251-260), so Lees-Edwards ghost-row
+// candidates are not included.  The cells of the last step (P.cell) are
+// counting-sorted per packing (rows 0..ny+1, ghost copies marked), then each
+// particle counts its 3x3 window.
+constexpr int kGhostBit = 1 << 30;
+constexpr int kObs = 8;            // per-particle observable terms
+
+__global__ void __launch_bounds__(kBlock)
+k_cell_count(Parts P, const Pack* __restrict__ packs, const int* __restrict__ rbase, int* cnt) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= P.n) return;
+    const int pk = P.pack[p];
+    const int nx = packs[pk].pnx;
+    const int2 c = P.cell[p];
+    atomicAdd(&cnt[rbase[pk] + (c.x >> 16) * nx + (c.x & 0xffff)], 1);
+    if (c.y >= 0) atomicAdd(&cnt[rbase[pk] + (c.y >> 16) * nx + (c.y & 0xffff)], 1);
+}
+
+// Exclusive prefix sum of cnt[0..m) into start[0..m], one workgroup of
+// kScan threads, each summing a contiguous chunk.
+constexpr int kScan = 1024;
+__global__ void __launch_bounds__(kScan) k_scan1(const int* cnt, int* start, int m) {
+    __shared__ int part[kScan];
+    const int t = threadIdx.x;
+    const int chunk = (m + kScan - 1) / kScan;
+    const int b = min(m, t * chunk), e = min(m, b + chunk);
+    int acc = 0;
+    for (int k = b; k < e; ++k) acc += cnt[k];
+    part[t] = acc;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int k = 0; k < kScan; ++k) { const int v = part[k]; part[k] = run; run += v; }
+        start[m] = run;
+    }
+    __syncthreads();
+    int run = part[t];
+    for (int k = b; k < e; ++k) { start[k] = run; run += cnt[k]; }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_cell_fill(Parts P, const Pack* __restrict__ packs, const int* __restrict__ rbase, int* cur,
+            int* list) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= P.n) return;
+    const int pk = P.pack[p];
+    const int nx = packs[pk].pnx;
+    const int2 c = P.cell[p];
+    list[atomicAdd(&cur[rbase[pk] + (c.x >> 16) * nx + (c.x & 0xffff)], 1)] = p;
+    if (c.y >= 0)
+        list[atomicAdd(&cur[rbase[pk] + (c.y >> 16) * nx + (c.y & 0xffff)], 1)] = p | kGhostBit;
+}
+
+// Per particle: kinetic energy (BaseParticle::getKineticEnergy,
+// src/BaseParticle.h:127), whether it is a rattler, its volume
+// (src/EllipticParticle.cpp:9, src/CircularParticle.cpp:18; a clump's mass:
+// density 1), its own contacts' count and elastic energy terms
+// (src/BaseWorld.cpp:271-284), and its normal-list candidates.
+__global__ void __launch_bounds__(kBlock)
+k_obs_particles(Parts P, const Pack* __restrict__ packs, Table t, const int* __restrict__ rbase,
+                const int* __restrict__ start, const int* __restrict__ list, double* out) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= P.n) return;
+    const Pack& K = packs[P.pack[p]];
+    const double vx = P.vx[p], vy = P.vy[p], w = P.w[p];
+    const double ke = 0.5 * (P.m[p] * (vx * vx + vy * vy) + P.inr[p] * w * w);
+    const bool rat = is_rattler(P, p);
+    double vol = 0.0;
+    if (!(P.flags[p] & SKIP_MACRO_SUM)) {
+        const double r = P.r[p];
+        vol = P.kind[p] == KIND_ELLIPSE ? M_PI * r * r / P.ar[p]
+            : P.kind[p] == KIND_CIRCLE ? M_PI * r * r : P.m[p];
+    }
+    double el = 0.0;
+    int nc = 0;
+    for (int e = t.row_start[p]; e < t.row_start[p] + t.row_cnt[p]; ++e) {
+        const int c = t.link[e];
+        if (c < 0) continue;
+        const double ox = t.fnx[c] / K.kn, oy = t.fny[c] / K.kn;
+        el += ox * ox + oy * oy;
+        ++nc;
+    }
+    int poss = 0;
+    const bool skip_i = K.world == WORLD_PERIODIC && (P.flags[p] & SKIP_CONTACT_DETECTION);
+    if (!skip_i) {
+        const int nx = K.pnx;
+        const int ci = P.cell[p].x;
+        const int row = ci >> 16, col = ci & 0xffff;
+        const int eidi = P.eid[p], upi = P.up[p];
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int r = row + dy;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int cx = col + dx;
+                if (cx == -1) cx = nx - 1;
+                else if (cx == nx) cx = 0;
+                const int b = rbase[P.pack[p]] + r * nx + cx;
+                for (int q = start[b]; q < start[b + 1]; ++q) {
+                    const int code = list[q];
+                    if ((code & kGhostBit) && K.world != WORLD_PERIODIC) continue;
+                    const int j = code & ~kGhostBit;
+                    if (eidi >= P.eid[j] || upi == P.up[j]) continue;
+                    if (K.world == WORLD_PERIODIC && (P.flags[j] & SKIP_CONTACT_DETECTION)) continue;
+                    ++poss;
+                }
+            }
+        }
+    }
+    double* o = out + (size_t)kObs * p;
+    o[0] = ke; o[1] = rat ? 0.0 : ke; o[2] = rat ? 0.0 : 1.0; o[3] = vol;
+    o[4] = el; o[5] = (double)nc; o[6] = (double)poss; o[7] = 0.0;
+}
+
+// One workgroup per packing: fixed-order sums of the per-particle terms and
+// the reference's derived values, out[10k..10k+9] as xdem_get_observables.
+__global__ void __launch_bounds__(kBlock)
+k_obs_sum(const Pack* __restrict__ packs, const double* terms, double* out) {
+    __shared__ double red[kObs - 1][kBlock];
+    const Pack& K = packs[blockIdx.x];
+    double a[kObs - 1] = {0, 0, 0, 0, 0, 0, 0};
+    for (int p = K.p_begin + (int)threadIdx.x; p < K.p_end; p += kBlock)
+        for (int k = 0; k < kObs - 1; ++k) a[k] += terms[(size_t)kObs * p + k];
+    for (int k = 0; k < kObs - 1; ++k) red[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < kObs - 1; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double n = (double)(K.p_end - K.p_begin);
+    const double ke = red[0][0], ken = red[1][0], nonr = red[2][0], vol = red[3][0];
+    const double el = red[4][0], actual = red[5][0], poss = red[6][0];
+    double* o = out + 10 * (size_t)blockIdx.x;
+    o[0] = poss;
+    o[1] = actual;
+    o[2] = ke;
+    o[3] = ke / n;
+    o[4] = nonr == 0.0 ? 0.0 : ken / nonr;
+    o[5] = el * K.kn * 0.5;
+    o[6] = actual == 0.0 ? 0.0 : el * K.kn * 0.5 / actual;
+    o[7] = vol;
+    o[8] = vol / (K.xlen * K.ylen);
+    o[9] = nonr;
 }
 
 // Fixed ellipse quantities of src/EllipticParticle.cpp:49-61, 72: b = a/AR,
@@ -2001,6 +2153,7 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         if (K.vlenx < vcell || K.vleny < vcell) K.skin = std::max(0.0, std::min(K.vlenx, K.vleny) - 2.0 * rmax);
         K.hx = K.xlen / 2.0; K.hy = K.ylen / 2.0;
         K.vnx_alloc = K.vnx; K.vny_alloc = K.vny;
+        K.pnx = K.nx; K.pny = K.ny;
         K.vcell = 2.0 * rmax + K.skin;      // after the skin fits the grid
         K.bin_base = bins_total;
         bins_total += (K.vny + 2) * K.vnx;
@@ -2077,6 +2230,11 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         return rc;
     HIPCHK(hipMemcpy(im_d, im.data(), db, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(iin_d, iin.data(), db, hipMemcpyHostToDevice));
+    {
+        double* inr_d = nullptr;
+        if ((rc = dalloc(E, &inr_d, n)) || (rc = up_f64(inr_d, inertia))) return rc;
+        P.inr = inr_d;
+    }
     {
         std::vector<int> v = permuted(up_level, perm);
         HIPCHK(hipMemcpy(up_d, v.data(), ib, hipMemcpyHostToDevice));
@@ -2536,6 +2694,50 @@ int xdem_get_box(xdem_engine* E, double* out) {
         o[4] = h[k].velx; o[5] = h[k].vely;
         for (int q = 0; q < 4; ++q) o[6 + q] = h[k].tstress[q];
     }
+    return XDEM_OK;
+}
+
+int xdem_get_observables(xdem_engine* E, double* out) {
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    if (!out) return fail(E, XDEM_E_INVALID, "out is NULL");
+    std::vector<Pack> h(E->npacks);
+    HIPCHK(hipMemcpy(h.data(), E->packs_d, sizeof(Pack) * E->npacks, hipMemcpyDeviceToHost));
+    std::vector<int> rbase(E->npacks + 1, 0);
+    for (int k = 0; k < E->npacks; ++k) rbase[k + 1] = rbase[k] + (h[k].pny + 2) * h[k].pnx;
+    const int cells = rbase[E->npacks];
+    // on-demand work arrays, released before returning
+    const size_t base = E->allocs.size();
+    int *rb_d, *cnt_d, *start_d, *cur_d, *list_d;
+    double* terms_d;
+    if ((rc = dalloc(E, &rb_d, E->npacks + 1)) || (rc = dalloc(E, &cnt_d, cells)) ||
+        (rc = dalloc(E, &start_d, cells + 1)) || (rc = dalloc(E, &cur_d, cells + 1)) ||
+        (rc = dalloc(E, &list_d, 2 * (size_t)E->n)) ||
+        (rc = dalloc(E, &terms_d, (size_t)kObs * E->n)))
+        return rc;
+    double* out_d;
+    if ((rc = dalloc(E, &out_d, 10 * (size_t)E->npacks))) return rc;
+    HIPCHK(hipMemcpyAsync(rb_d, rbase.data(), sizeof(int) * (E->npacks + 1), hipMemcpyHostToDevice,
+                          E->stream));
+    HIPCHK(hipMemsetAsync(cnt_d, 0, sizeof(int) * (size_t)cells, E->stream));
+    const int gn = grid_of(E->n);
+    hipLaunchKernelGGL(k_cell_count, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
+                       rb_d, cnt_d);
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(kScan), 0, E->stream, cnt_d, start_d, cells);
+    HIPCHK(hipMemcpyAsync(cur_d, start_d, sizeof(int) * (size_t)(cells + 1),
+                          hipMemcpyDeviceToDevice, E->stream));
+    hipLaunchKernelGGL(k_cell_fill, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
+                       rb_d, cur_d, list_d);
+    hipLaunchKernelGGL(k_obs_particles, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
+                       E->T[E->cur], rb_d, start_d, list_d, terms_d);
+    hipLaunchKernelGGL(k_obs_sum, dim3(E->npacks), dim3(kBlock), 0, E->stream, E->packs_d,
+                       terms_d, out_d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, out_d, sizeof(double) * 10 * E->npacks, hipMemcpyDeviceToHost,
+                          E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    for (size_t k = base; k < E->allocs.size(); ++k) hipFree(E->allocs[k]);
+    E->allocs.resize(base);
     return XDEM_OK;
 }
 

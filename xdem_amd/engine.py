@@ -27,7 +27,7 @@ SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles
            "xdem_set_contacts", "xdem_step", "xdem_sync", "xdem_get_particles",
            "xdem_get_forces", "xdem_get_contacts", "xdem_get_stress", "xdem_get_boundary",
            "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
-           "xdem_set_control", "xdem_get_box",
+           "xdem_set_control", "xdem_get_box", "xdem_get_observables",
            "xdem_set_mlp", "xdem_get_mlp", "xdem_mlp_init", "xdem_log_roots",
            "xdem_log_count", "xdem_log_fetch", "xdem_mlp_train", "xdem_mlp_grad",
            "xdem_mlp_predict")
@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_get_traffic.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.xdem_set_control.argtypes = [vp, C.c_int, C.POINTER(XdemControl)]
     L.xdem_get_box.argtypes = [vp, _D]
+    L.xdem_get_observables.argtypes = [vp, _D]
     L.xdem_set_mlp.argtypes = [vp, _F]
     L.xdem_get_mlp.argtypes = [vp, _F]
     L.xdem_mlp_init.argtypes = [vp, C.c_uint]
@@ -236,6 +237,16 @@ class Engine:
         stress of the last control step; shape (packings, 10)."""
         out = np.zeros(10 * len(self.states))
         self._check(self.L.xdem_get_box(self.h, out))
+        return out.reshape(-1, 10)
+
+    def observables(self):
+        """Per packing (shape (packings, 10)): countPossibleContacts,
+        countActucalContacts, getKineticEnergy, getKineticEnergyPerParticle,
+        getKineticEnergyPerNonRattlerParticle, getElasticEnergy,
+        getElasticEnergyPerContact, total particle volume, getSolidFraction,
+        non-rattler count (xdem_get_observables)."""
+        out = np.zeros(10 * len(self.states))
+        self._check(self.L.xdem_get_observables(self.h, out))
         return out.reshape(-1, 10)
 
     def boundary(self):

@@ -1,8 +1,12 @@
 // Driver loop written against the reference-style adapter xdem_world.hpp:
 // reads particles from a text file (one line per particle:
 // kind r AR x y theta vx vy omega), runs N takeTimeStep calls of a
-// Lees-Edwards (or periodic) world and prints contacts, stress and shift.
-//   world_demo <file> <le:0|1> <xmin> <xmax> <ymin> <ymax> <dt> <rate> <steps>
+// Lees-Edwards, periodic or stress-controlled periodic world and prints
+// contacts, stress, observables and the box.
+//   world_demo <file> <world> <xmin> <xmax> <ymin> <ymax> <dt> <rate> <steps>
+//   world: 0 periodic, 1 Lees-Edwards (rate = shear rate), 2 periodic with
+//   the drivers' compression loop (stress control X/Y at stress = rate,
+//   rattler velocities scaled by 0.99 before each step)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -19,7 +23,8 @@ int main(int argc, char** argv) {
     while (in >> p.kind >> p.outerRadius >> p.aspectRatio >> p.x >> p.y >> p.rotation >> p.vx >>
            p.vy >> p.rotationVel)
         parts.push_back(p);
-    const bool le = atoi(argv[2]) != 0;
+    const int kind = atoi(argv[2]);
+    const bool le = kind == 1;
     const double dt = atof(argv[7]), rate = atof(argv[8]);
     const int steps = atoi(argv[9]);
     xdem::ContactModelParams cm{1e4, 5e3, 0.5, 0.5};
@@ -30,14 +35,28 @@ int main(int argc, char** argv) {
             lw->setWorldBoundary(atof(argv[3]), atof(argv[4]), atof(argv[5]), atof(argv[6]));
             lw->setShiftStrainRate(rate);
             w.reset(lw);
+        } else if (kind == 2) {
+            auto* cw = new xdem::GpuPeriodicControlWorld(dt, parts, cm);
+            cw->setWorldBoundary(atof(argv[3]), atof(argv[4]), atof(argv[5]), atof(argv[6]));
+            cw->setVirtualBoundaryMass();
+            cw->scaleVelocityOfRattlersEachStep(0.99);
+            cw->stressControlXYEachStep(rate);
+            w.reset(cw);
         } else {
             w.reset(new xdem::GpuPeriodicBoundaryWorld(dt, parts, cm));
             w->setWorldBoundary(atof(argv[3]), atof(argv[4]), atof(argv[5]), atof(argv[6]));
         }
         for (int s = 0; s < steps; ++s) w->takeTimeStep();
         const xdem::matrix2 st = w->updateTotalStress();
-        printf("{\"contacts\": %d, \"stress\": [%.17g, %.17g, %.17g, %.17g], \"ke\": %.17g}\n",
-               w->countActucalContacts(), st.xx, st.xy, st.yx, st.yy, w->getKineticEnergy());
+        double box[4] = {0, 0, 0, 0};
+        if (kind == 2) {
+            const std::vector<double> b = static_cast<xdem::GpuPeriodicControlWorld*>(w.get())->box();
+            for (int k = 0; k < 4; ++k) box[k] = b[k];
+        }
+        printf("{\"contacts\": %d, \"possible\": %d, \"stress\": [%.17g, %.17g, %.17g, %.17g], "
+               "\"ke\": %.17g, \"solid_fraction\": %.17g, \"box\": [%.17g, %.17g, %.17g, %.17g]}\n",
+               w->countActucalContacts(), w->countPossibleContacts(), st.xx, st.xy, st.yx, st.yy,
+               w->getKineticEnergy(), w->getSolidFraction(), box[0], box[1], box[2], box[3]);
     } catch (const xdem::Error& e) {
         fprintf(stderr, "xdem error %d: %s\n", e.code, e.what());
         return 1;

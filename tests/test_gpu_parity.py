@@ -327,3 +327,36 @@ def test_separation_pretest_keeps_contact_decisions(world):
     print(f"[gpu] pre-test {world}: {ce['i'].size} contacts of {len(poses['a1'])} pairs", flush=True)
     assert ce["i"].size > 0.3 * len(poses["a1"])
     assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"])
+
+
+def test_cpp_adapter_control_world(tmp_path):
+    """The adapter's GpuPeriodicControlWorld (the drivers' compression loop:
+    rattler scaling, stress control X/Y) against the Python binding."""
+    import json
+    import os
+    import subprocess
+    from tests.golden_io import load_control
+    demo = os.path.join(os.path.dirname(__file__), "cpp", "world_demo")
+    s, mode, kw, _ = load_control("stress_xy")
+    s.vx[:] = 0.0
+    s.vy[:] = 0.0
+    s.omega[:] = 0.0
+    f = tmp_path / "parts.txt"
+    with open(f, "w") as fh:
+        for k in range(s.n):
+            vals = [float(v) for v in (s.r[k], s.aspect[k], s.x[k], s.y[k], s.theta[k])]
+            fh.write(f"{int(s.kind[k])} " + " ".join(repr(v) for v in vals) + " 0 0 0\n")
+    c = s.cfg
+    args = [repr(float(v)) for v in (c.xmin, c.xmax, c.ymin, c.ymax, c.dt, kw["stress"])]
+    r = subprocess.run([demo, str(f), "2", *args, "8"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    eng = _engine(s)
+    eng.set_control(mode, stress=kw["stress"], rattler_scale=0.99)
+    eng.step(8)
+    obs = eng.observables()[0]
+    assert out["contacts"] == int(obs[1]) and out["possible"] == int(obs[0])
+    assert np.array_equal(np.array(out["stress"]), eng.stress()[0])
+    assert np.array_equal(np.array(out["box"]), eng.box()[0][:4])
+    assert out["solid_fraction"] == obs[8]

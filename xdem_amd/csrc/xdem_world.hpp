@@ -109,29 +109,15 @@ public:
         return stress_;
     }
     const matrix2 getStress() const { return stress_; }
-    const int countActucalContacts() {         // src/BaseWorld.cpp:262-269
-        upload();
-        int nc = 0;
-        check(xdem_get_contacts(e_, 0, &nc, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                nullptr, nullptr, nullptr, nullptr));
-        return nc;
-    }
-    const double getKineticEnergy() {          // src/BaseWorld.cpp:302-310
-        sync_state();
-        double k = 0.0;
-        for (const auto& p : parts_)
-            k += 0.5 * (p.mass * (p.vx * p.vx + p.vy * p.vy) + p.inertia * p.rotationVel * p.rotationVel);
-        return k;
-    }
-    const double getSolidFraction() const {    // src/BaseWorld.cpp:331-334
-        double v = 0.0;
-        for (const auto& p : parts_)
-            if (!(p.settings & XDEM_SKIP_MACRO_SUM))
-                v += p.kind == XDEM_KIND_CIRCLE ? M_PI * p.outerRadius * p.outerRadius
-                                                : M_PI * p.outerRadius * p.outerRadius / p.aspectRatio;
-        return v / ((cfg_.xmax - cfg_.xmin) * (cfg_.ymax - cfg_.ymin));
-    }
+    // observables, computed on the device (src/BaseWorld.cpp:251-334)
+    const int countPossibleContacts() { return (int)observable(0); }
+    const int countActucalContacts() { return (int)observable(1); }
+    const double getKineticEnergy() { return observable(2); }
+    const double getKineticEnergyPerParticle() { return observable(3); }
+    const double getKineticEnergyPerNonRattlerParticle() { return observable(4); }
+    const double getElasticEnergy() { return observable(5); }
+    const double getElasticEnergyPerContact() { return observable(6); }
+    const double getSolidFraction() { return observable(8); }
     // particle state in ID order (BaseParticle getters)
     const std::vector<ParticleSpec>& particles() { sync_state(); return parts_; }
     void getForces(std::vector<double>& fx, std::vector<double>& fy, std::vector<double>& tq) {
@@ -141,6 +127,12 @@ public:
     }
 
 protected:
+    double observable(int k) {
+        upload();
+        double o[10];
+        check(xdem_get_observables(e_, o));
+        return o[k];
+    }
     void upload() {
         if (e_) return;
         check(xdem_create(&e_, 1, &cfg_));
@@ -157,6 +149,8 @@ protected:
         check(xdem_set_particles(e_, n, nullptr, kind.data(), up.data(), fl.data(), r.data(),
                                  ar.data(), m.data(), I.data(), x.data(), y.data(), th.data(),
                                  vx.data(), vy.data(), w.data(), bx.data(), by.data()));
+        if (ctl_.mode || ctl_.reset_rattlers || ctl_.rattler_scale != 1.0)
+            check(xdem_set_control(e_, 0, &ctl_));
     }
     void sync_state() {
         if (!e_ || !dirty_) return;
@@ -174,6 +168,7 @@ protected:
     }
 
     xdem_config cfg_{};
+    xdem_control ctl_{XDEM_CONTROL_NONE, 0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     xdem_engine* e_ = nullptr;
     std::vector<ParticleSpec> parts_;
     matrix2 stress_{0, 0, 0, 0};
@@ -185,6 +180,44 @@ class GpuPeriodicBoundaryWorld : public GpuWorld {
 public:
     GpuPeriodicBoundaryWorld(double dt, const std::vector<ParticleSpec>& p, const ContactModelParams& cm)
         : GpuWorld(XDEM_WORLD_PERIODIC, dt, p, cm) {}
+};
+
+// PeriodicControlWorld(dt, particleHandlers, contactModel), src/PeriodicControlWorld.h:15-20.
+// The reference's drivers call the boundary update and the rattler treatment
+// between the step's parts (mainIsoElpsPack_WCCM.cpp:127-137); here they are
+// configured once and then run inside every takeTimeStep, on the device, in
+// that same order.  Configure before the first step.
+class GpuPeriodicControlWorld : public GpuWorld {
+public:
+    GpuPeriodicControlWorld(double dt, const std::vector<ParticleSpec>& p, const ContactModelParams& cm)
+        : GpuWorld(XDEM_WORLD_PERIODIC, dt, p, cm) {}
+    void setVirtualBoundaryMass() { ctl_.boundary_mass = 0.0; }            // :58-66
+    void setVirtualBoundaryMass(double mass) { ctl_.boundary_mass = mass; } // :68-71
+    void setMaxStrainRate(double rate) { ctl_.max_strain_rate = rate; }    // :73-76
+    // scaleVelocityOfRattlers(scale) before every step (src/BaseWorld.cpp:94-106)
+    void scaleVelocityOfRattlersEachStep(double scale) { ctl_.rattler_scale = scale; }
+    // resetVelocityOfRattlers after every integration, as PeriodicControlWorld::takeTimeStep
+    void resetVelocityOfRattlersEachStep(bool on = true) { ctl_.reset_rattlers = on ? 1 : 0; }
+    // updateTotalStress + the boundary update after every step
+    void strainControlEachStep(double rateX, double rateY) {          // :124-135
+        ctl_.mode = XDEM_CONTROL_STRAIN; ctl_.strain_rate_x = rateX; ctl_.strain_rate_y = rateY;
+    }
+    void stressControlUniformEachStep(double stress) {               // :137-151
+        ctl_.mode = XDEM_CONTROL_STRESS_UNIFORM; ctl_.stress = stress;
+    }
+    void stressControlXYEachStep(double stress) {                    // :153-193
+        ctl_.mode = XDEM_CONTROL_STRESS_XY; ctl_.stress = stress;
+    }
+    void stressControlUniformStrainLimitEachStep(double stress) {    // :213-237
+        ctl_.mode = XDEM_CONTROL_STRESS_LIMITED; ctl_.stress = stress;
+    }
+    // xmin, xmax, ymin, ymax, boundary velocity x, y, control stress (4)
+    std::vector<double> box() {
+        upload();
+        std::vector<double> b(10);
+        check(xdem_get_box(e_, b.data()));
+        return b;
+    }
 };
 
 // LeesEdwardsBoundaryWorld(dt, particleHandlers, contactModel), src/LeesEdwardsBoundaryWorld.h:18-26

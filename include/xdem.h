@@ -230,6 +230,21 @@ int xdem_get_box(xdem_engine* e, double* out);
  * elastic energy and rattlers refer to the last step.  Computed on the device. */
 int xdem_get_observables(xdem_engine* e, double* out);
 
+/* ---- Time-history files ----------------------------------------------------
+ * The text files of the reference's writers, for its post-processing
+ * (QuadronAnalysis, ClusterLabelling) and MATLAB viewers.  Formats as
+ * writeParticleInformation2Files (src/BaseWorld.cpp:336-357) and
+ * writeParticleTimeHistory2Files (src/PeriodicBoundaryWorld.cpp:192-211,
+ * src/LeesEdwardsBoundaryWorld.cpp:385-409); IDs are indices inside the
+ * packing; names are "<dir>/output_*.dat" (the reference joins with '\\');
+ * contacts of one particle come in ascending partner order. */
+/* writeParticleInformation2Files + createParticleTimeHistoryFiles */
+int xdem_history_open(xdem_engine* e, int packing, const char* dir);
+/* writeParticleTimeHistory2Files; the stress line is that of the current contacts */
+int xdem_history_write(xdem_engine* e, int packing);
+/* closeParticleTimeHistoryFiles (src/BaseWorld.cpp:377-391) */
+int xdem_history_close(xdem_engine* e, int packing);
+
 /* ---- Learned quartic-root guess (XDEM_PREC_NN) --------------------------
  * No counterpart in the reference, whose Newton iteration always starts at
  * s = sqrt(delta + 1) (NewtonRaphsonForQuartic, src/EllipticParticle.cpp:163,

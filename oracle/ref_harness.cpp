@@ -19,6 +19,11 @@
 //       step, resetVelocityOfRattlers, updateTotalStress, boundary update
 //       (mode 1 strain, 2 stress uniform, 3 stress X then Y, 4 stress uniform
 //       with strain limit); dumps every step plus the boundary record.
+//   ref_harness history <state.xst> <prefix> <nsteps> <every>
+//       the reference's own file writers: writeParticleInformation2Files and
+//       createParticleTimeHistoryFiles(prefix), then every `every` steps
+//       updateTotalStress + writeParticleTimeHistory2Files.  The reference
+//       joins names with '\\', so the files are "<prefix>\\output_*.dat".
 //
 // The state file layout is written by xdem_amd/state.py (write_state).
 #include <chrono>
@@ -342,6 +347,28 @@ int mode_bench(int argc, char** argv) {
     return 0;
 }
 
+int mode_history(int argc, char** argv) {
+    if (argc < 6) return 2;
+    Built B;
+    B.s = read_state(argv[2]);
+    const std::string prefix = argv[3];
+    const int nsteps = atoi(argv[4]), every = atoi(argv[5]);
+    build(B, 1, 100);
+    BaseWorld* W = B.world();
+    W->writeParticleInformation2Files(prefix);
+    W->createParticleTimeHistoryFiles(prefix);
+    for (int s = 1; s <= nsteps; ++s) {
+        W->takeTimeStep();
+        if (s % every == 0) {
+            W->updateTotalStress();
+            W->writeParticleTimeHistory2Files();
+        }
+    }
+    W->flushAllFiles();
+    W->closeParticleTimeHistoryFiles();
+    return 0;
+}
+
 int mode_control(int argc, char** argv) {
     if (argc < 13) return 2;
     Built B;
@@ -394,5 +421,6 @@ int main(int argc, char** argv) {
     if (m == "pairs") return mode_pairs(argc, argv);
     if (m == "bench") return mode_bench(argc, argv);
     if (m == "control") return mode_control(argc, argv);
+    if (m == "history") return mode_history(argc, argv);
     return 2;
 }

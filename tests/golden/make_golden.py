@@ -17,6 +17,9 @@ Fixtures
                     update; src/PeriodicControlWorld.cpp:124-237): state,
                     contacts, stress and the boundary record after steps
                     1, 5 and 20
+  history_<case>.npz  the text files of the reference's own writers
+                    (writeParticleInformation2Files,
+                    writeParticleTimeHistory2Files after steps 3 and 6)
   pairs.npz         calculateContactGeometry of the reference on random
                     ellipse and circle poses (src/EllipticParticle.cpp:16-285,
                     src/CircularParticle.cpp:67-92)
@@ -101,6 +104,29 @@ def run_control(name, tmp):
     print(f"control {name}: n={s.n} box after {CONTROL_STEPS} steps {steps[-1]['box'][:4]}")
 
 
+HISTORY_CASES = ("le_poly", "pb_circle", "pb_clumps")
+HISTORY_STEPS, HISTORY_EVERY = 6, 3
+
+
+def run_history(name, tmp):
+    import glob
+    s = helpers.state(name)
+    sp = os.path.join(tmp, f"hist_{name}.xst")
+    write_state(sp, s)
+    d = os.path.join(tmp, f"hist_{name}")
+    os.makedirs(d)
+    prefix = os.path.join(d, "run")
+    subprocess.run([po.HARNESS, "history", sp, prefix, str(HISTORY_STEPS), str(HISTORY_EVERY)],
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=d,
+                   timeout=120)
+    out = {"in_" + k: v for k, v in state_arrays(s).items()}
+    for f in glob.glob(prefix + "\\output_*.dat"):
+        key = os.path.basename(f).split("\\")[-1][:-4]
+        out["file_" + key] = np.frombuffer(open(f, "rb").read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, f"history_{name}.npz"), **out)
+    print(f"history {name}: {sorted(k for k in out if k.startswith('file_'))}")
+
+
 def run_pairs(tmp, m=3000, seed=42):
     rng = np.random.default_rng(seed)
     rows = np.zeros((m, 12))
@@ -132,6 +158,8 @@ def main():
             run_case(c, tmp)
         for c in CONTROL_CASES:
             run_control(c, tmp)
+        for c in HISTORY_CASES:
+            run_history(c, tmp)
         run_pairs(tmp)
 
 

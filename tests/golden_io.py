@@ -52,3 +52,21 @@ def load_control(name):
         pre = f"s{st}_"
         steps[st] = {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
     return s, int(z["mode"][0]), kw, steps
+
+
+HISTORY_CASES = ("le_poly", "pb_circle", "pb_clumps")
+
+
+def load_history(name):
+    """(state, {file name: text}) of the reference's writers after steps 3
+    and 6 (tests/golden/make_golden.py:run_history)."""
+    z = np.load(os.path.join(GOLDEN, f"history_{name}.npz"))
+    cfgv = z["in_cfg"]
+    cfg = WorldConfig(world_kind=int(cfgv[0]), **{k: float(v) for k, v in zip(_CFG_KEYS, cfgv[1:])})
+    f = lambda k: z["in_" + k].copy()
+    s = State(cfg, *(f(k) for k in ("kind", "up_level", "flags", "r", "aspect", "mass",
+                                     "inertia", "x", "y", "theta", "vx", "vy", "omega",
+                                     "clump_bx", "clump_by", "ci", "cj", "xi_x", "xi_y",
+                                     "n_x", "n_y", "damping")))
+    files = {k[5:]: z[k].tobytes().decode() for k in z.files if k.startswith("file_")}
+    return s, files

@@ -1892,6 +1892,8 @@ struct xdem_engine {
     // boundary control / rattler treatment set on some packing
     bool control_active = false;
     double* box_stress_d = nullptr;   // 4 per packing
+    // time-history files per packing (createParticleTimeHistoryFiles)
+    std::vector<std::vector<FILE*>> hist;
 };
 
 namespace {
@@ -2064,6 +2066,8 @@ int xdem_create(xdem_engine** out, int num_packings, const xdem_config* cfgs) {
 
 void xdem_destroy(xdem_engine* E) {
     if (!E) return;
+    for (auto& fs : E->hist)
+        for (FILE* f : fs) fclose(f);
     if (E->stream) hipStreamSynchronize(E->stream);
     for (int k = 0; k < 2; ++k) {
         if (E->graph_exec[k]) hipGraphExecDestroy(E->graph_exec[k]);
@@ -2739,6 +2743,128 @@ int xdem_get_observables(xdem_engine* E, double* out) {
     for (size_t k = base; k < E->allocs.size(); ++k) hipFree(E->allocs[k]);
     E->allocs.resize(base);
     return XDEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Time-history and shape files in the formats of the reference's writers, so
+// that its post-processing (QuadronAnalysis, src/QuadronAnalysis.cpp:388-440;
+// ClusterLabelling, src/ClusterLabelling.cpp:55-131) and MATLAB viewers read
+// GPU runs.  IDs are indices inside the packing.  File names join the
+// directory with '/' where the reference writes a Windows '\\' separator;
+// contacts of one particle are listed by ascending partner ID (the
+// reference lists them in its unordered_map's order).
+
+int xdem_history_open(xdem_engine* E, int packing, const char* dir) {
+    if (!E || !E->have_particles) return fail(E, XDEM_E_INVALID, "set particles first");
+    if (!dir || packing < 0 || packing >= E->npacks) return fail(E, XDEM_E_INVALID, "bad packing or directory");
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    if (E->hist.size() != (size_t)E->npacks) E->hist.resize(E->npacks);
+    for (FILE* f : E->hist[packing]) fclose(f);
+    E->hist[packing].clear();
+    const std::string d(dir);
+    // writeParticleInformation2Files (src/BaseWorld.cpp:336-357): radius and
+    // aspect ratio, "%lf\n" each (EllipticParticle.h:70-71,
+    // CircularParticle.h:54-55, GluedParticle.h:49-50)
+    std::vector<double> r(E->n), ar(E->n);
+    std::vector<int8_t> k8(E->n);
+    if ((rc = fetch_ext(E, r.data(), E->P.r)) || (rc = fetch_ext(E, ar.data(), E->P.ar)) ||
+        (rc = fetch_ext(E, k8.data(), E->P.kind)))
+        return rc;
+    FILE* fr = fopen((d + "/output_radius.dat").c_str(), "w");
+    FILE* fa = fopen((d + "/output_aspectRatio.dat").c_str(), "w");
+    if (!fr || !fa) {
+        if (fr) fclose(fr);
+        if (fa) fclose(fa);
+        return fail(E, XDEM_E_INVALID, "cannot create the shape files in " + d);
+    }
+    for (int p = E->pack_start[packing]; p < E->pack_start[packing + 1]; ++p) {
+        const bool clump = k8[p] == KIND_CLUMP, circle = k8[p] == KIND_CIRCLE;
+        fprintf(fr, "%lf\n", clump ? 0.0 : r[p]);
+        fprintf(fa, "%lf\n", (clump || circle) ? 1.0 : ar[p]);
+    }
+    fclose(fr);
+    fclose(fa);
+    // createParticleTimeHistoryFiles (src/PeriodicBoundaryWorld.cpp:144-190,
+    // src/LeesEdwardsBoundaryWorld.cpp:335-383)
+    std::vector<std::string> names = {"output_velocity", "output_position", "output_omega",
+                                      "output_rotation", "output_contact", "output_boundary",
+                                      "output_kineticEnergy"};
+    if (E->packs_h[packing].world == WORLD_LE) names.push_back("output_boundaryShift");
+    names.push_back("output_stress");
+    for (const std::string& nm : names) {
+        FILE* f = fopen((d + "/" + nm + ".dat").c_str(), "w");
+        if (!f) {
+            for (FILE* q : E->hist[packing]) fclose(q);
+            E->hist[packing].clear();
+            return fail(E, XDEM_E_INVALID, "cannot create " + nm + ".dat in " + d);
+        }
+        E->hist[packing].push_back(f);
+    }
+    return XDEM_OK;
+}
+
+// writeParticleTimeHistory2Files (src/PeriodicBoundaryWorld.cpp:192-211,
+// src/LeesEdwardsBoundaryWorld.cpp:385-409) for one packing.  The stress
+// line is updateTotalStress of the current contacts.
+int xdem_history_write(xdem_engine* E, int packing) {
+    if (!E || packing < 0 || packing >= (int)E->hist.size() || E->hist[packing].empty())
+        return fail(E, XDEM_E_INVALID, "history files are not open for this packing");
+    const int b = E->pack_start[packing], e = E->pack_start[packing + 1];
+    std::vector<double> x(E->n), y(E->n), th(E->n), vx(E->n), vy(E->n), w(E->n);
+    int rc = xdem_get_particles(E, x.data(), y.data(), th.data(), vx.data(), vy.data(), w.data());
+    if (rc) return rc;
+    std::vector<double> ob(10 * (size_t)E->npacks), st(4 * (size_t)E->npacks), box(10 * (size_t)E->npacks),
+        lb(3 * (size_t)E->npacks);
+    if ((rc = xdem_get_observables(E, ob.data())) || (rc = xdem_get_stress(E, st.data())) ||
+        (rc = xdem_get_box(E, box.data())) || (rc = xdem_get_boundary(E, lb.data())))
+        return rc;
+    int nc = 0;
+    if ((rc = xdem_get_contacts(E, 0, &nc, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, nullptr)))
+        return rc;
+    std::vector<int> ci(nc), cj(nc);
+    std::vector<double> fnx(nc), fny(nc), fsx(nc), fsy(nc), px(nc), py(nc);
+    if ((rc = xdem_get_contacts(E, nc, &nc, ci.data(), cj.data(), fnx.data(), fny.data(),
+                                fsx.data(), fsy.data(), px.data(), py.data(), nullptr, nullptr,
+                                nullptr, nullptr, nullptr, nullptr, nullptr)))
+        return rc;
+    std::vector<FILE*>& f = E->hist[packing];
+    for (int p = b; p < e; ++p) fprintf(f[0], "%le\t%le\t", vx[p], vy[p]);
+    fprintf(f[0], "\n");
+    for (int p = b; p < e; ++p) fprintf(f[1], "%le\t%le\t", x[p], y[p]);
+    fprintf(f[1], "\n");
+    for (int p = b; p < e; ++p) fprintf(f[2], "%le\t", w[p]);
+    fprintf(f[2], "\n");
+    for (int p = b; p < e; ++p) fprintf(f[3], "%le\t", th[p]);
+    fprintf(f[3], "\n");
+    int mine = 0;
+    for (int k = 0; k < nc; ++k) mine += ci[k] >= b && ci[k] < e;
+    fprintf(f[4], "%d\n", mine);
+    for (int k = 0; k < nc; ++k) {                 // (i, j) ascending
+        if (ci[k] < b || ci[k] >= e) continue;
+        fprintf(f[4], "%d\t%d\t%le\t%le\t%le\t%le\t%le\t%le\n", ci[k] - b, cj[k] - b, px[k],
+                py[k], fnx[k], fny[k], fsx[k], fsy[k]);
+    }
+    const double* bx = &box[10 * (size_t)packing];
+    fprintf(f[5], "%le\t%le\t%le\t%le\n", bx[0], bx[1], bx[2], bx[3]);
+    fprintf(f[6], "%le\n", ob[10 * (size_t)packing + 3]);
+    size_t k = 7;
+    if (E->packs_h[packing].world == WORLD_LE) fprintf(f[k++], "%le\n", lb[3 * (size_t)packing]);
+    const double* s4 = &st[4 * (size_t)packing];
+    fprintf(f[k], "%le\t%le\t%le\t%le\n", s4[0], s4[1], s4[2], s4[3]);
+    return XDEM_OK;
+}
+
+// flushAllFiles / closeParticleTimeHistoryFiles (src/BaseWorld.cpp:377-391)
+int xdem_history_close(xdem_engine* E, int packing) {
+    if (!E || packing < 0 || packing >= (int)E->hist.size())
+        return fail(E, XDEM_E_INVALID, "history files are not open for this packing");
+    int bad = 0;
+    for (FILE* f : E->hist[packing]) bad |= fclose(f) != 0;
+    E->hist[packing].clear();
+    return bad ? fail(E, XDEM_E_INVALID, "closing a history file failed") : XDEM_OK;
 }
 
 int xdem_get_counters(xdem_engine* E, long long* newton_iters, long long* rescues,

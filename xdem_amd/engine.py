@@ -28,6 +28,7 @@ SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles
            "xdem_get_forces", "xdem_get_contacts", "xdem_get_stress", "xdem_get_boundary",
            "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
            "xdem_set_control", "xdem_get_box", "xdem_get_observables",
+           "xdem_history_open", "xdem_history_write", "xdem_history_close",
            "xdem_set_mlp", "xdem_get_mlp", "xdem_mlp_init", "xdem_log_roots",
            "xdem_log_count", "xdem_log_fetch", "xdem_mlp_train", "xdem_mlp_grad",
            "xdem_mlp_predict")
@@ -88,6 +89,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_set_control.argtypes = [vp, C.c_int, C.POINTER(XdemControl)]
     L.xdem_get_box.argtypes = [vp, _D]
     L.xdem_get_observables.argtypes = [vp, _D]
+    L.xdem_history_open.argtypes = [vp, C.c_int, C.c_char_p]
+    L.xdem_history_write.argtypes = [vp, C.c_int]
+    L.xdem_history_close.argtypes = [vp, C.c_int]
     L.xdem_set_mlp.argtypes = [vp, _F]
     L.xdem_get_mlp.argtypes = [vp, _F]
     L.xdem_mlp_init.argtypes = [vp, C.c_uint]
@@ -248,6 +252,18 @@ class Engine:
         out = np.zeros(10 * len(self.states))
         self._check(self.L.xdem_get_observables(self.h, out))
         return out.reshape(-1, 10)
+
+    def history_open(self, directory: str, packing: int = 0):
+        """Shape files now and time-history files for history_write
+        (writeParticleInformation2Files + createParticleTimeHistoryFiles)."""
+        self._check(self.L.xdem_history_open(self.h, int(packing), directory.encode()))
+
+    def history_write(self, packing: int = 0):
+        """writeParticleTimeHistory2Files of the current state."""
+        self._check(self.L.xdem_history_write(self.h, int(packing)))
+
+    def history_close(self, packing: int = 0):
+        self._check(self.L.xdem_history_close(self.h, int(packing)))
 
     def boundary(self):
         out = np.zeros(3 * len(self.states))

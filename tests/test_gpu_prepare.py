@@ -1,0 +1,105 @@
+"""Packing preparation (xdem_amd/prepare.py) on the GPU engine.
+
+Parity: the first steps of the relaxation (overlapping circles, rattler
+velocities reset after every step) and of the compression (the relaxed
+circles turned into ellipses under stress control in x and y) against the
+CPU oracle on the same states: identical contact sets, positions within
+1e-10 absolute and velocities within 1e-8 of their scale (the contact and
+stress sums run in another order on the device; the stress-controlled
+boundary feeds that last-bit difference back into every position).
+Whole protocol at a small size: the relaxation ends below 20 contacts, the
+compression jams the ellipses under the target pressure."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from xdem_amd import prepare as pp
+
+pytestmark = pytest.mark.gpu
+
+N, AR = 300, 1.5
+
+
+def _compare(eng, orc, what, st, pos_tol=1e-10, vel_rel=1e-8):
+    ce, co = eng.contacts(), orc.contacts()
+    assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"]), \
+        f"{what}: contact set differs at step {st}"
+    pe, po_ = eng.particles(), orc.particles()
+    for k in ("x", "y", "theta"):
+        assert float(np.max(np.abs(pe[k] - po_[k]))) <= pos_tol, f"{what}: {k} step {st}"
+    for k in ("vx", "vy", "omega"):
+        sc = max(float(np.max(np.abs(po_[k]))), 1e-300)
+        assert float(np.max(np.abs(pe[k] - po_[k]))) <= vel_rel * sc, f"{what}: {k} step {st}"
+    be, bo = eng.box()[0], orc.box()
+    assert np.all(np.abs(be[:4] - bo[:4]) <= pos_tol), f"{what}: box step {st}"
+
+
+def _relaxed_by_oracle(s):
+    orc = po.OracleWorld(s)
+    orc.set_control(**pp.relax_control())
+    for _ in range(500):
+        orc.step(100)
+        if orc.observables()[1] < 20:
+            break
+    return orc.export_state(s)
+
+
+def test_relaxation_steps_match_oracle():
+    from xdem_amd.engine import Engine
+    s = pp.reference_generation(N, AR)
+    print(f"[gpu] relaxation parity n={s.n}", flush=True)
+    eng = Engine(s)
+    eng.set_control(**pp.relax_control())
+    orc = po.OracleWorld(s)
+    orc.set_control(**pp.relax_control())
+    done = 0
+    for st in (1, 10, 50):
+        eng.step(st - done)
+        orc.step(st - done)
+        done = st
+        _compare(eng, orc, "relax", st)
+    ob_e, ob_o = eng.observables()[0], orc.observables()
+    assert ob_e[1] == ob_o[1] and ob_e[1] > 0
+    eng.close()
+
+
+def test_compression_steps_match_oracle():
+    from xdem_amd.engine import Engine
+    r = _relaxed_by_oracle(pp.reference_generation(N, AR))
+    e = pp.ellipses_from_circles(r, AR, friction=0.5, restitution=0.01)
+    print(f"[gpu] compression parity n={e.n}", flush=True)
+    eng = Engine(e)
+    eng.set_control(**pp.compression_control(1e4))
+    orc = po.OracleWorld(e)
+    orc.set_control(**pp.compression_control(1e4))
+    done = 0
+    for st in (1, 20, 200, 1000):
+        eng.step(st - done)
+        orc.step(st - done)
+        done = st
+        if st < 1000:
+            _compare(eng, orc, "compress", st)
+        else:
+            _compare(eng, orc, "compress", st, pos_tol=1e-8, vel_rel=1e-6)
+    assert eng.box()[0][1] < e.cfg.xmax and eng.observables()[0][1] > 10
+    eng.close()
+
+
+def test_protocol_small():
+    s0 = pp.reference_generation(256, AR)
+    print("[gpu] preparation protocol n=256", flush=True)
+    r, steps = pp.relax(s0, check_every=100)
+    assert steps < 50_000 and r.ci.shape[0] < 20
+    c, csteps = pp.compress(r, AR, 1e5, friction=0.5, restitution=0.01,
+                            min_steps=20_000, max_steps=20_000, check_every=5_000)
+    assert csteps == 20_000
+    # ellipses of aspect ratio 1.5 cover 0.8 / 1.5 of the box after the
+    # relaxation; at 1e5 (overlaps ~1e-2 of the radius at kn = 1e7) the
+    # stress control jams them near 0.9
+    area = (c.cfg.xmax - c.cfg.xmin) * (c.cfg.ymax - c.cfg.ymin)
+    phi = float(np.sum(np.pi * c.r * c.r / c.aspect)) / area
+    assert phi > 0.85
+    assert c.cfg.xmax == -c.cfg.xmin and c.cfg.ymax == -c.cfg.ymin
+    assert c.ci.shape[0] > c.n
+    for k in ("x", "y", "vx", "vy", "omega"):
+        assert np.all(np.isfinite(getattr(c, k)))

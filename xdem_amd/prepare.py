@@ -1,0 +1,222 @@
+"""Packing preparation on the GPU engine, following the protocol of the
+reference's drivers (mainIsoElpsPack_WCCM.cpp:21-160):
+
+1. random generation (:46-73): circles with radius U(2/3, 4/3) * sqrt(AR),
+   centres uniform in [-100, 100]^2 with overlap, orientation U(0, 2 pi),
+   density 10, then the box and the positions scaled to solid fraction 0.8
+   (PeriodicBoundaryWorld::scaleBoundarySizeToGivenSolidFraction,
+   src/PeriodicBoundaryWorld.cpp:276-284);
+2. relaxation (:74-91): periodic world, kn = kt = 1e7, restitution 0.95, no
+   friction, resetVelocityOfRattlers after every step, until fewer than 20
+   contacts remain (at most 50,000 steps);
+3. compression (:97-155): the circles become ellipses of the given aspect
+   ratio (EllipticParticle(const CircularParticle&, ...), src/EllipticParticle.h:32-38)
+   in a PeriodicControlWorld with the given friction and restitution, global
+   damping 0.1 and the virtual boundary mass; every step scales the rattlers'
+   velocities by 0.99 and controls the stress in x and then y at the target
+   pressure, until the kinetic energy per particle is below 1e-10 after more
+   than 100,000 steps.
+
+The random numbers are the reference's: std::default_random_engine (minstd_rand0)
+seeded with 0, and uniform_real_distribution<double> as libstdc++ computes it
+(generate_canonical<double, 53> from two draws), consumed in the order g++
+evaluates the driver's expressions (the constructor's arguments right to
+left: orientation, y, x, radius; the unused aspect-ratio draw first).
+Stages 2 and 3 check their stop criteria every `check_every` steps rather
+than every step.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .engine import CONTROL_NONE, CONTROL_STRESS_XY
+from .state import (KIND_CIRCLE, KIND_ELLIPSE, WORLD_PERIODIC, State, WorldConfig,
+                    particle_properties)
+
+_M = 2147483647
+
+
+class MinstdRand0:
+    """std::minstd_rand0: x <- 16807 x mod (2^31 - 1); seed 0 gives x = 1."""
+
+    def __init__(self, seed: int = 0):
+        s = seed % _M
+        self.x = 1 if s == 0 else s
+
+    def __call__(self) -> int:
+        self.x = (16807 * self.x) % _M
+        return self.x
+
+
+def canonical(eng: MinstdRand0) -> float:
+    """std::generate_canonical<double, 53> (libstdc++) over minstd_rand0:
+    range r = max - min + 1 = 2^31 - 2, floor(log2 r) = 30, so two draws:
+    (g1 - 1) + (g2 - 1) r, divided by fl(r^2)."""
+    r = 2147483646.0
+    s, t = 0.0, 1.0
+    for _ in range(2):
+        s += float(eng() - 1) * t
+        t *= r
+    u = s / t
+    return u if u < 1.0 else math.nextafter(1.0, 0.0)
+
+
+def uniform(eng: MinstdRand0, a: float, b: float) -> float:
+    """std::uniform_real_distribution<double>(a, b)(eng), libstdc++: u (b - a) + a."""
+    return canonical(eng) * (b - a) + a
+
+
+def reference_draws(n: int, aspect_ratio: float, seed: int = 0, half_box: float = 100.0):
+    """The random values of mainIsoElpsPack_WCCM.cpp:46-66 in the order g++
+    consumes them: per particle alpha, then the CircularParticle constructor's
+    arguments right to left (orientation, y, x, radius).  Returns arrays
+    (alpha, radius, x, y, orientation)."""
+    eng = MinstdRand0(seed)
+    f = math.sqrt(aspect_ratio)
+    al = np.empty(n); r = np.empty(n); x = np.empty(n); y = np.empty(n); th = np.empty(n)
+    for i in range(n):
+        al[i] = uniform(eng, 1.01, aspect_ratio)
+        th[i] = uniform(eng, 0.0, math.pi * 2.0)
+        y[i] = uniform(eng, -half_box, half_box)
+        x[i] = uniform(eng, -half_box, half_box)
+        r[i] = uniform(eng, 2.0 / 3.0, 4.0 / 3.0) * f
+    return al, r, x, y, th
+
+
+def reference_generation(n: int, aspect_ratio: float, seed: int = 0, half_box: float = 100.0,
+                         solid_fraction: float = 0.8, density: float = 10.0) -> State:
+    """Stage 1: the circles of mainIsoElpsPack_WCCM.cpp:46-73, in a box scaled
+    to the initial solid fraction."""
+    _, r, x, y, th = reference_draws(n, aspect_ratio, seed, half_box)
+    kind = np.full(n, KIND_CIRCLE, np.int32)
+    aspect = np.ones(n)
+    mass, inertia, vol = particle_properties(kind, r, aspect, density)
+    # updateTotalParticleVolume (src/BaseWorld.cpp:188-199), index order
+    total = 0.0
+    for v in vol:
+        total += float(v)
+    xmin, xmax, ymin, ymax = -half_box, half_box, -half_box, half_box
+    current = total / ((xmax - xmin) * (ymax - ymin))
+    scale = math.sqrt(current / solid_fraction)
+    x = x * scale
+    y = y * scale
+    xmin *= scale; xmax *= scale; ymin *= scale; ymax *= scale   # scaleBoundarySize
+    cfg = WorldConfig(world_kind=WORLD_PERIODIC, xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax,
+                      dt=1e-4, kn=1e7, kt=1e7, restitution=0.95, mu=0.0)
+    z = np.zeros(n)
+    return State(cfg, kind, np.arange(n, dtype=np.int32), np.zeros(n, np.int32), r, aspect,
+                 mass, inertia, x, y, th, z.copy(), z.copy(), z.copy(), z.copy(), z.copy())
+
+
+def relax_control() -> dict:
+    """set_control arguments of stage 2: no boundary control,
+    resetVelocityOfRattlers after every step."""
+    return dict(mode=CONTROL_NONE, reset_rattlers=True)
+
+
+def compression_control(pressure: float) -> dict:
+    """set_control arguments of stage 3: scaleVelocityOfRattlers(0.99) before
+    every step, updatePeriodicBoundary_stressControlX then ...Y at `pressure`
+    after it (mainIsoElpsPack_WCCM.cpp:125-136)."""
+    return dict(mode=CONTROL_STRESS_XY, stress=float(pressure), rattler_scale=0.99)
+
+
+def ellipses_from_circles(s: State, aspect_ratio: float, friction: float, restitution: float,
+                          density: float = 10.0, global_damping: float = 0.1) -> State:
+    """The stage-3 packing (mainIsoElpsPack_WCCM.cpp:97-122): every circle
+    becomes an ellipse with semi-major axis equal to its radius and the given
+    aspect ratio, same pose and velocity (src/EllipticParticle.h:32-38), mass
+    and inertia recomputed at `density`; no contact history; the contact
+    model takes the new friction and restitution and the global damping."""
+    n = s.n
+    e = s.copy()
+    e.kind = np.full(n, KIND_ELLIPSE, np.int32)
+    e.aspect = np.full(n, float(aspect_ratio))
+    e.mass, e.inertia, _ = particle_properties(e.kind, e.r, e.aspect, density)
+    e = e.with_history(np.zeros(0, np.int32), np.zeros(0, np.int32),
+                       *[np.zeros(0) for _ in range(5)])
+    e.cfg.mu = float(friction)
+    e.cfg.restitution = float(restitution)
+    e.cfg.global_damping = float(global_damping)
+    return e
+
+
+def _read_back(eng, s: State) -> State:
+    out = s.copy()
+    for key, v in eng.particles().items():
+        setattr(out, key, v)
+    box = eng.box()[0]
+    out.cfg.xmin, out.cfg.xmax, out.cfg.ymin, out.cfg.ymax = (float(v) for v in box[:4])
+    c = eng.contacts()
+    return out.with_history(c["i"], c["j"], c["xi_x"], c["xi_y"], c["n_x"], c["n_y"],
+                            c["damping"])
+
+
+def relax(s: State, max_steps: int = 50_000, check_every: int = 100, min_contacts: int = 20,
+          log=None) -> tuple[State, int]:
+    """Stage 2 (mainIsoElpsPack_WCCM.cpp:74-91): steps with
+    resetVelocityOfRattlers until fewer than `min_contacts` contacts remain,
+    tested every `check_every` steps.  Returns the relaxed state and the
+    steps taken."""
+    from .engine import Engine
+    eng = Engine(s)
+    try:
+        eng.set_control(**relax_control())
+        steps = 0
+        while steps < max_steps:
+            k = min(check_every, max_steps - steps)
+            eng.step(k)
+            steps += k
+            nc = int(eng.observables()[0][1])
+            if log:
+                log(f"[relax] step {steps}: {nc} contacts")
+            if nc < min_contacts:
+                break
+        return _read_back(eng, s), steps
+    finally:
+        eng.close()
+
+
+def compress(s: State, aspect_ratio: float, pressure: float, friction: float,
+             restitution: float, density: float = 10.0, min_steps: int = 100_000,
+             max_steps: int = 2_000_000, check_every: int = 1000, ke_stop: float = 1e-10,
+             log=None) -> tuple[State, int]:
+    """Stage 3 (mainIsoElpsPack_WCCM.cpp:97-155) from the relaxed circles:
+    stress control in x and y at `pressure` until the kinetic energy per
+    particle is below `ke_stop` after more than `min_steps` steps (tested
+    every `check_every` steps), or `max_steps`.  Returns the compressed
+    state (box, particles, contact history) and the steps taken."""
+    from .engine import Engine
+    e = ellipses_from_circles(s, aspect_ratio, friction, restitution, density)
+    eng = Engine(e)
+    try:
+        eng.set_control(**compression_control(pressure))
+        steps = 0
+        while steps < max_steps:
+            k = min(check_every, max_steps - steps)
+            eng.step(k)
+            steps += k
+            ke = float(eng.observables()[0][3])
+            if log:
+                log(f"[compress] step {steps}: kinetic energy per particle {ke:.3e}")
+            if ke < ke_stop and steps > min_steps:
+                break
+        return _read_back(eng, e), steps
+    finally:
+        eng.close()
+
+
+def prepare_packing(n: int, aspect_ratio: float, pressure: float, friction: float,
+                    restitution: float = 0.01, seed: int = 0, relax_max_steps: int = 50_000,
+                    compress_min_steps: int = 100_000, compress_max_steps: int = 2_000_000,
+                    check_every: int = 1000, log=None) -> State:
+    """The three stages of the reference's drivers on the GPU engine: a
+    jammed packing of `n` ellipses at `pressure`."""
+    s = reference_generation(n, aspect_ratio, seed)
+    s, _ = relax(s, max_steps=relax_max_steps, log=log)
+    s, _ = compress(s, aspect_ratio, pressure, friction, restitution,
+                    min_steps=compress_min_steps, max_steps=compress_max_steps,
+                    check_every=check_every, log=log)
+    return s

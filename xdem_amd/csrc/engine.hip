@@ -949,7 +949,7 @@ template <int PREC>
 __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, const Pack& K,
                                              const Table& old_t, const Table& new_t, int e,
                                              int lst, int found, int i, int j, float guess,
-                                             uint32_t* err, StepCounts& n) {
+                                             bool record, uint32_t* err, StepCounts& n) {
     const double xi = P.x[i], yi = P.y[i];
     double ox, oy, ovx;
     image_offset(K, lst, xi, yi, P.x[j], P.y[j], ox, oy, ovx);
@@ -1128,26 +1128,39 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
     FRec r;
     r.fx = fx; r.fy = fy; r.t1 = t1; r.t2 = t2;
     new_t.fr[ct] = r;
-    new_t.fnx[ct] = fnx; new_t.fny[ct] = fny;
-    new_t.fsx[ct] = fsx; new_t.fsy[ct] = fsy;
-    new_t.bx[ct] = xi - xj2;
-    new_t.by[ct] = yi - yj2;
-    new_t.px[ct] = g.px; new_t.py[ct] = g.py;
+    // Detail fields: read only by the readers after a call, by the stress sum
+    // of the boundary control and by a carry-forward (copy_contact).  A pair
+    // in contact now is closer than r_i + r_j; by the next step its distance
+    // grows by less than the skin (the pair-list rebuild test bounds the
+    // displacements and the Lees-Edwards shift change by it), and two centres
+    // closer than a cell length lie in adjacent cells, so the pair stays a
+    // candidate and cannot be carried forward unless
+    // r_i + r_j + skin >= the shorter cell length.
+    if (record || P.r[i] + P.r[j] + K.skin >= fmin(K.lenx, K.leny)) {
+        new_t.fnx[ct] = fnx; new_t.fny[ct] = fny;
+        new_t.fsx[ct] = fsx; new_t.fsy[ct] = fsy;
+        new_t.bx[ct] = xi - xj2;
+        new_t.by[ct] = yi - yj2;
+        new_t.px[ct] = g.px; new_t.py[ct] = g.py;
+    }
 }
 
 // One lane per work item.
 template <int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t,
-          WorkList W, uint32_t* err, Counters* cnt) {
+          WorkList W, const int* __restrict__ record_flag, uint32_t* err, Counters* cnt) {
     __shared__ int s_pre[kSeg + 1];
     __shared__ int s_go;
+    __shared__ int s_rec;
     if (threadIdx.x == 0) {
         s_go = *err == 0;
+        s_rec = *record_flag;
         segment_prefix(s_pre, W.counter, W.seg_cap);
     }
     __syncthreads();
     if (!s_go) return;
+    const bool record = s_rec != 0;
     int wbeg, wend;
     xcd_chunk(s_pre[kSeg], wbeg, wend);
     StepCounts n;
@@ -1161,7 +1174,7 @@ k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table n
         const int i = it.z, j = it.w;
         const float guess = PREC == kNN ? W.guess[at] : 0.0f;
         const Pack K = packs[P.pack[i]];
-        contact_item<PREC>(P, F, K, old_t, new_t, e, lst, it.y, i, j, guess, err, n);
+        contact_item<PREC>(P, F, K, old_t, new_t, e, lst, it.y, i, j, guess, record, err, n);
     }
     Counters* cs = counter_slot(cnt);
     count_add(&cs->newton, n.newton);
@@ -1790,6 +1803,10 @@ k_obs_sum(const Pack* __restrict__ packs, const double* terms, double* out) {
 
 // Fixed ellipse quantities of src/EllipticParticle.cpp:49-61, 72: b = a/AR,
 // e^2, b^2, b/a - 1, a/b - 1 (circles and clumps keep zeros).
+__global__ void k_set_flag(int* p, int v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
 __global__ void k_shape(int n, const double* r, const double* ar, const int8_t* kind,
                         double* b, double* e2, double* bsq, double* s1, double* eta) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1863,6 +1880,13 @@ struct xdem_engine {
     Counters* cnt_d = nullptr;
     double* stress_d = nullptr;
     int* rebuild_d = nullptr;     // two alternating pair-list rebuild flags
+    // 1: the contact kernel writes every contact's detail fields (normal and
+    // tangential force, branch vector, contact point); 0: only where a later
+    // step may carry the contact forward.  Set to 1 for the last step of each
+    // xdem_step call (what the readers see) and for every step while boundary
+    // control is active (the stress sum needs the branch vectors).
+    int* record_d = nullptr;
+    int record_state = -1;        // value last written to record_d
     long long steps = 0;
     bool have_particles = false;
     bool timing = false;
@@ -2317,9 +2341,10 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         return rc;
     if ((rc = dalloc(E, &E->packs_d, E->npacks)) || (rc = dalloc(E, &E->err_d, 1)) ||
         (rc = dalloc(E, &E->cnt_d, kSeg)) || (rc = dalloc(E, &E->stress_d, (size_t)4 * n)) ||
-        (rc = dalloc(E, &E->rebuild_d, 2)) ||
+        (rc = dalloc(E, &E->rebuild_d, 2)) || (rc = dalloc(E, &E->record_d, 1)) ||
         (rc = dalloc(E, &E->box_stress_d, (size_t)4 * E->npacks)))
         return rc;
+    E->record_state = -1;
     HIPCHK(hipMemcpy(E->packs_d, E->packs_h.data(), sizeof(Pack) * E->npacks, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(E->err_d, 0, sizeof(uint32_t)));
     HIPCHK(hipMemset(E->cnt_d, 0, sizeof(Counters) * kSeg));
@@ -2436,7 +2461,7 @@ int launch_step_t(xdem_engine* E) {
                            E->mlp_pk, E->err_d);
     if (E->timing) HIPCHK(hipEventRecord(e0, E->stream));
     hipLaunchKernelGGL(k_contact<PREC>, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
-                       E->F, E->packs_d, old_t, new_t, E->W, E->err_d, E->cnt_d);
+                       E->F, E->packs_d, old_t, new_t, E->W, E->record_d, E->err_d, E->cnt_d);
     if (E->timing) HIPCHK(hipEventRecord(e1, E->stream));
     if (E->logging)
         hipLaunchKernelGGL(k_log, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P, E->F,
@@ -2497,15 +2522,28 @@ int xdem_step(xdem_engine* E, int nsteps) {
         return fail(E, XDEM_E_INVALID, "learned-guess mode: set or train the network first");
     int left = nsteps;
     int rc;
+    // contact details: every step under boundary control, else the last
+    // step (or the last two, when they run as one graph)
+    auto set_record = [&](int v) -> int {
+        if (E->record_state == v) return XDEM_OK;
+        hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, E->stream, E->record_d, v);
+        HIPCHK(hipGetLastError());
+        E->record_state = v;
+        return XDEM_OK;
+    };
+    if (nsteps > 0 && (rc = set_record(E->control_active ? 1 : 0))) return rc;
     while (left >= 2 && E->use_graphs && !E->timing && !E->logging) {
         const int key = (int)(E->steps & 1);
         if (!E->graph_exec[key] && (rc = capture_pair(E))) return rc;
+        if (left == 2 && (rc = set_record(1))) return rc;
         HIPCHK(hipGraphLaunch(E->graph_exec[key], E->stream));
         E->steps += 2;                // two flips of E->cur cancel
         left -= 2;
     }
-    for (; left > 0; --left)
+    for (; left > 0; --left) {
+        if (left == 1 && (rc = set_record(1))) return rc;
         if ((rc = launch_step(E))) return rc;
+    }
     return XDEM_OK;
 }
 

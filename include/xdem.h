@@ -150,6 +150,17 @@ int xdem_get_contacts(xdem_engine* e, int cap, int* nc, int* i, int* j,
                       double* xi_x, double* xi_y, double* n_x, double* n_y,
                       double* damping, double* torque_i, double* torque_j);
 
+/* Branch vectors of the same contacts, same order: _branchVector (centre of
+ * P1 minus the image centre of P2), _branchVectorFromP1 and
+ * _branchVectorFromP2 (contact point minus each centre), as the contact
+ * computation left them (src/ContactInformation.h:15-29; set in
+ * BaseParticle::calculateContactForceAndTorque, src/BaseParticle.cpp:27-98,
+ * the branch vector at :85).
+ * What the text checkpoint prints besides xdem_get_contacts' fields. */
+int xdem_get_contact_branches(xdem_engine* e, int cap, int* nc, double* branch_x,
+                              double* branch_y, double* b1_x, double* b1_y,
+                              double* b2_x, double* b2_y);
+
 /* Stress of each packing, BaseWorld::updateTotalStress
  * (src/BaseWorld.cpp:158-176): out[4k..4k+3] = xx, xy, yx, yy. */
 int xdem_get_stress(xdem_engine* e, double* out);

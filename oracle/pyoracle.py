@@ -49,6 +49,7 @@ def lib():
         L.oracle_get_forces.argtypes = [C.c_void_p, _D, _D, _D, _I]
         L.oracle_num_contacts.argtypes = [C.c_void_p]
         L.oracle_get_contacts.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _I, _I] + [_D] * 13
+        L.oracle_get_branches.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)] + [_D] * 6
         L.oracle_get_stress.argtypes = [C.c_void_p, _D]
         L.oracle_get_boundary.argtypes = [C.c_void_p, _D]
         L.oracle_get_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_longlong)] * 5
@@ -127,6 +128,16 @@ class OracleWorld:
         d = dict(zip(names, arrs))
         d["i"], d["j"] = ci, cj
         return d
+
+    def contact_branches(self):
+        """Branch vectors of the contacts() entries, same order."""
+        L = lib()
+        m = L.oracle_num_contacts(self.h)
+        names = ("branch_x", "branch_y", "b1_x", "b1_y", "b2_x", "b2_y")
+        arrs = [np.zeros(m) for _ in names]
+        nc = C.c_int(0)
+        L.oracle_get_branches(self.h, m, C.byref(nc), *arrs)
+        return dict(zip(names, arrs))
 
     def stress(self):
         out = np.zeros(4)

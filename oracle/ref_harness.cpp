@@ -24,10 +24,19 @@
 //       createParticleTimeHistoryFiles(prefix), then every `every` steps
 //       updateTotalStress + writeParticleTimeHistory2Files.  The reference
 //       joins names with '\\', so the files are "<prefix>\\output_*.dat".
+//   ref_harness checkpoint <state.xst> <ckpt.txt> <restart.xdg> <k> <m>
+//       k steps, then the text checkpoint BaseWorld::print2File_allParticles
+//       (src/BaseWorld.cpp:369-375); a second world is built from that file
+//       with the reference's stream constructor EllipticParticle(std::ifstream&,
+//       aspectRatio, contactModel, ifReadContact = true) (src/EllipticParticle.h:40-48,
+//       read4File_particle src/BaseParticle.cpp:173-186), the boundary and
+//       Lees-Edwards shift of the first, and runs m steps, dumped after each.
+//       Ellipses only (the circle's stream constructor sets no contact model).
 //
 // The state file layout is written by xdem_amd/state.py (write_state).
 #include <chrono>
 #include <cstdio>
+#include <fstream>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -147,6 +156,8 @@ struct Built {
     }
 };
 
+void make_world(Built& B, int threads, int chunk, bool control);
+
 void build(Built& B, int threads, int chunk, bool control = false) {
     State& s = B.s;
     const double* c = s.cfg;
@@ -189,6 +200,12 @@ void build(Built& B, int threads, int chunk, bool control = false) {
         else
             ((HEllipse*)p)->inject(s.cj[k], s.xix[k], s.xiy[k], s.nx[k], s.ny[k], s.c[k]);
     }
+    make_world(B, threads, chunk, control);
+}
+
+void make_world(Built& B, int threads, int chunk, bool control) {
+    const State& s = B.s;
+    const double* c = s.cfg;
     if (s.world == 1) {
         B.le.reset(new HLE(c[4], B.handles, B.cm.get()));
         B.le->setWorldBoundary(c[0], c[1], c[2], c[3]);
@@ -369,6 +386,47 @@ int mode_history(int argc, char** argv) {
     return 0;
 }
 
+int mode_checkpoint(int argc, char** argv) {
+    if (argc < 7) return 2;
+    Built B;
+    B.s = read_state(argv[2]);
+    const int k = atoi(argv[5]), m = atoi(argv[6]);
+    for (int i = 0; i < B.s.n; ++i)
+        if (B.s.kind[i] != 1) { fprintf(stderr, "checkpoint: ellipses only\n"); return 2; }
+    build(B, 1, 100);
+    for (int s = 1; s <= k; ++s) B.world()->takeTimeStep();
+    FILE* t = fopen(argv[3], "w");
+    if (!t) return 3;
+    B.world()->print2File_allParticles(t);
+    fclose(t);
+    double sc[3] = {0, 0, 0};
+    if (B.le) B.le->scalars(sc);
+    Built R;
+    R.s = B.s;
+    const double* c = R.s.cfg;
+    R.cm.reset(new ContactModel(c[5], c[6], c[7], c[8]));
+    std::ifstream is(argv[3]);
+    int n = 0;
+    is >> n;
+    if (n != R.s.n) return 3;
+    for (int i = 0; i < n; ++i) {
+        auto* q = new HEllipse(is, R.s.ar[i], R.cm.get(), true);
+        R.parts.emplace_back(q);
+        R.handles.push_back(q);
+    }
+    make_world(R, 1, 100, false);
+    if (R.le) R.le->set_shift(sc[0], sc[2]);
+    FILE* f = fopen(argv[4], "wb");
+    if (!f) return 3;
+    fwrite("XDEMGD02", 1, 8, f);
+    for (int s = 1; s <= m; ++s) {
+        R.world()->takeTimeStep();
+        dump(R, f, s);
+    }
+    fclose(f);
+    return 0;
+}
+
 int mode_control(int argc, char** argv) {
     if (argc < 13) return 2;
     Built B;
@@ -422,5 +480,6 @@ int main(int argc, char** argv) {
     if (m == "bench") return mode_bench(argc, argv);
     if (m == "control") return mode_control(argc, argv);
     if (m == "history") return mode_history(argc, argv);
+    if (m == "checkpoint") return mode_checkpoint(argc, argv);
     return 2;
 }

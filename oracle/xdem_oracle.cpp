@@ -930,6 +930,28 @@ int oracle_num_contacts(void* h) {
 }
 
 // live contacts ordered by (i, j)
+// Branch vectors of the contacts, in oracle_get_contacts' order
+// (ContactInformation::_branchVector, _branchVectorFromP1/P2,
+// src/ContactInformation.h:18-20, set in src/BaseParticle.cpp:85-86).
+int oracle_get_branches(void* h, int cap, int* nc, double* bx, double* by, double* b1x,
+                        double* b1y, double* b2x, double* b2y) {
+    World& W = *(World*)h;
+    int k = 0;
+    for (int i = 0; i < W.n; ++i) {
+        for (auto& kv : W.hist[i]) {
+            if (k < cap) {
+                const Contact& c = kv.second;
+                bx[k] = c.bx; by[k] = c.by;
+                b1x[k] = c.b1x; b1y[k] = c.b1y;
+                b2x[k] = c.b2x; b2y[k] = c.b2y;
+            }
+            ++k;
+        }
+    }
+    *nc = k;
+    return 0;
+}
+
 int oracle_get_contacts(void* h, int cap, int* nc, int* ci, int* cj,
                         double* fnx, double* fny, double* fsx, double* fsy,
                         double* px, double* py, double* xix, double* xiy,

@@ -5,7 +5,9 @@ oracle/Makefile into oracle/_ref/; this script drives it through
 oracle/_ref/ref_harness and stores inputs and outputs as compressed npz
 files.  Run it in a container where /root/reference is present:
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [group ...]
+
+with groups world, control, history, checkpoint, pairs (default: all).
 
 Fixtures
   world_<case>.npz  initial state (+ optional history) and the reference's
@@ -20,6 +22,10 @@ Fixtures
   history_<case>.npz  the text files of the reference's own writers
                     (writeParticleInformation2Files,
                     writeParticleTimeHistory2Files after steps 3 and 6)
+  checkpoint_<case>.npz  the text checkpoint BaseWorld::print2File_allParticles
+                    after 4 steps, and a world rebuilt from it by the
+                    reference's stream constructors, dumped after each of 3
+                    further steps
   pairs.npz         calculateContactGeometry of the reference on random
                     ellipse and circle poses (src/EllipticParticle.cpp:16-285,
                     src/CircularParticle.cpp:67-92)
@@ -127,6 +133,28 @@ def run_history(name, tmp):
     print(f"history {name}: {sorted(k for k in out if k.startswith('file_'))}")
 
 
+CHECKPOINT_CASES = ("le_poly", "pb_ellipse")
+CHECKPOINT_K, CHECKPOINT_M = 4, 3
+
+
+def run_checkpoint(name, tmp):
+    s = helpers.state(name)
+    sp = os.path.join(tmp, f"ck_{name}.xst")
+    tp = os.path.join(tmp, f"ck_{name}.txt")
+    gp = os.path.join(tmp, f"ck_{name}.xdg")
+    write_state(sp, s)
+    subprocess.run([po.HARNESS, "checkpoint", sp, tp, gp, str(CHECKPOINT_K), str(CHECKPOINT_M)],
+                   check=True, stdout=subprocess.DEVNULL, timeout=120)
+    out = {"in_" + k: v for k, v in state_arrays(s).items()}
+    out["file_ckpt"] = np.frombuffer(open(tp, "rb").read(), np.uint8)
+    for d in po.read_golden(gp):
+        for k, v in d.items():
+            if k != "step":
+                out[f"r{d['step']}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, f"checkpoint_{name}.npz"), **out)
+    print(f"checkpoint {name}: {len(out['file_ckpt'])} bytes of text")
+
+
 def run_pairs(tmp, m=3000, seed=42):
     rng = np.random.default_rng(seed)
     rows = np.zeros((m, 12))
@@ -153,14 +181,22 @@ def main():
     if not po.have_reference():
         sys.exit("oracle/_ref/ref_harness missing: run `make -C oracle` where the "
                  "reference sources are present")
+    groups = set(sys.argv[1:]) or {"world", "control", "history", "checkpoint", "pairs"}
     with tempfile.TemporaryDirectory() as tmp:
-        for c in CASES:
-            run_case(c, tmp)
-        for c in CONTROL_CASES:
-            run_control(c, tmp)
-        for c in HISTORY_CASES:
-            run_history(c, tmp)
-        run_pairs(tmp)
+        if "world" in groups:
+            for c in CASES:
+                run_case(c, tmp)
+        if "control" in groups:
+            for c in CONTROL_CASES:
+                run_control(c, tmp)
+        if "history" in groups:
+            for c in HISTORY_CASES:
+                run_history(c, tmp)
+        if "checkpoint" in groups:
+            for c in CHECKPOINT_CASES:
+                run_checkpoint(c, tmp)
+        if "pairs" in groups:
+            run_pairs(tmp)
 
 
 if __name__ == "__main__":

@@ -70,3 +70,26 @@ def load_history(name):
                                      "n_x", "n_y", "damping")))
     files = {k[5:]: z[k].tobytes().decode() for k in z.files if k.startswith("file_")}
     return s, files
+
+
+CHECKPOINT_CASES = ("le_poly", "pb_ellipse")
+CHECKPOINT_K, CHECKPOINT_M = 4, 3
+
+
+def load_checkpoint(name):
+    """(state, text of print2File_allParticles after CHECKPOINT_K steps,
+    {step: dump of the world the reference rebuilt from that text})
+    (tests/golden/make_golden.py:run_checkpoint)."""
+    z = np.load(os.path.join(GOLDEN, f"checkpoint_{name}.npz"))
+    cfgv = z["in_cfg"]
+    cfg = WorldConfig(world_kind=int(cfgv[0]), **{k: float(v) for k, v in zip(_CFG_KEYS, cfgv[1:])})
+    f = lambda k: z["in_" + k].copy()
+    s = State(cfg, *(f(k) for k in ("kind", "up_level", "flags", "r", "aspect", "mass",
+                                     "inertia", "x", "y", "theta", "vx", "vy", "omega",
+                                     "clump_bx", "clump_by", "ci", "cj", "xi_x", "xi_y",
+                                     "n_x", "n_y", "damping")))
+    steps = {}
+    for st in range(1, CHECKPOINT_M + 1):
+        pre = f"r{st}_"
+        steps[st] = {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
+    return s, z["file_ckpt"].tobytes().decode(), steps

@@ -1,0 +1,144 @@
+"""Text checkpoint in the reference's format.
+
+Writer -- BaseWorld::print2File_allParticles (src/BaseWorld.cpp:369-375):
+the particle count, then for every particle in ID order
+BaseParticle::print2File_particle (src/BaseParticle.cpp:158-171; format
+src/BaseParticle.cpp:6-7): ID, outer radius, position, velocity, rotation,
+rotation velocity, volume, mass, 1/mass, inertia, 1/inertia and the number
+of contacts it holds, followed by each held contact as
+ContactInformation::print2File_contactInfo (src/ContactInformation.h:7-9,
+81-97): partner ID, contact normal, contact point, branch vector, branch
+vectors from P1 and P2, shear overlap, normal force, shear force, force,
+force on P1, force on P2, torque on P1, torque on P2, damping constant.
+A contact is held by P1, the lower ID of the pair.  The reference prints a
+particle's contacts in its unordered_map's order; they are printed here in
+ascending partner ID.
+
+Reader -- read4File_particle (src/BaseParticle.cpp:173-186) as the stream
+constructors use it (EllipticParticle, src/EllipticParticle.h:40-48;
+CircularParticle, src/CircularParticle.cpp:10-14): every particle gets
+upLevel = ID, its mass and inertia are taken from the file, and its
+contacts restart from the printed shear overlap, normal and damping
+constant.  Besides the base layout above (14 fields) the reader accepts the
+derived layouts of CircularParticle::print2File_particle (15 fields, the
+radius after the outer radius; src/CircularParticle.cpp:7-8, 53-65) and
+EllipticParticle::print2File_particle (16 fields, radius and aspect ratio;
+src/EllipticParticle.cpp:508-521), which the reference's own reader does
+not parse.  The file holds no world: box, contact model and Lees-Edwards
+shift come from the caller.  The values are printed with 9 significant
+digits; the reference's restart integrates with the printed 1/mass and
+1/inertia, this engine with the reciprocals of the printed mass and
+inertia (they differ by up to ~1e-8 relative).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .state import KIND_CIRCLE, KIND_CLUMP, KIND_ELLIPSE, State, WorldConfig
+
+_PARTICLE = "%7d\t" + "\t".join(["%.8e"] * 12) + "\t%4d\n"
+_CONTACT = "\t%7d\t" + "\t".join(["%.8e"] * 25) + "\n"
+
+
+def _volume(kind: int, r: float, ar: float, mass: float, density: float) -> float:
+    # EllipticParticle / CircularParticle::calculateParticleProperties
+    # (src/EllipticParticle.cpp:9, src/CircularParticle.cpp:18); a clump's
+    # volume is its integrated mass over the density (src/GluedParticle.cpp:39-55)
+    if kind == KIND_ELLIPSE:
+        return math.pi * r * r / ar
+    if kind == KIND_CIRCLE:
+        return math.pi * r * r
+    return mass / density
+
+
+def write_all_particles(path: str, s: State, contacts: dict, branches: dict,
+                        density: float = 1.0) -> None:
+    """Write `s` (particle state) with its contacts (a dict as
+    Engine.contacts(): i, j, n_x, ..., ordered by (i, j)) and their branch
+    vectors (Engine.contact_branches()) in the reference's text format."""
+    n = s.n
+    ci = np.asarray(contacts["i"])
+    start = np.searchsorted(ci, np.arange(n + 1))
+    g = lambda k: np.asarray(contacts[k], np.float64)
+    b = lambda k: np.asarray(branches[k], np.float64)
+    cols = [g("n_x"), g("n_y"), g("point_x"), g("point_y"), b("branch_x"), b("branch_y"),
+            b("b1_x"), b("b1_y"), b("b2_x"), b("b2_y"), g("xi_x"), g("xi_y"),
+            g("fn_x"), g("fn_y"), g("fs_x"), g("fs_y")]
+    fx, fy = g("fn_x") + g("fs_x"), g("fn_y") + g("fs_y")   # calculateTotalForceAndTorque
+    cols += [fx, fy, fx, fy, -fx, -fy, g("torque_i"), g("torque_j"), g("damping")]
+    cj = np.asarray(contacts["j"])
+    with open(path, "w") as f:
+        f.write("%d\n" % n)
+        for p in range(n):
+            m, inr = float(s.mass[p]), float(s.inertia[p])
+            vol = _volume(int(s.kind[p]), float(s.r[p]), float(s.aspect[p]), m, density)
+            c0, c1 = int(start[p]), int(start[p + 1])
+            f.write(_PARTICLE % (p, s.r[p], s.x[p], s.y[p], s.vx[p], s.vy[p], s.theta[p],
+                                 s.omega[p], vol, m, 1.0 / m, inr, 1.0 / inr, c1 - c0))
+            for c in range(c0, c1):
+                f.write(_CONTACT % ((int(cj[c]),) + tuple(float(a[c]) for a in cols)))
+
+
+def engine_checkpoint(eng, path: str, density: float = 1.0) -> None:
+    """print2File_allParticles of a one-packing Engine after its last step."""
+    if len(eng.states) != 1:
+        raise ValueError("one packing per text checkpoint")
+    s = eng.states[0].copy()
+    for k, v in eng.particles().items():
+        setattr(s, k, v)
+    write_all_particles(path, s, eng.contacts(), eng.contact_branches(), density)
+
+
+def read_all_particles(path: str, cfg: WorldConfig, aspect_ratio=None,
+                       kind: str = "ellipse") -> State:
+    """A State from a text checkpoint.  `kind` ("ellipse" or "circle") and
+    `aspect_ratio` (scalar or per particle) play the role of the stream
+    constructor's arguments; a 16-field line carries its own aspect ratio."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    n = int(lines[0])
+    pos = 1
+    fields = np.zeros((n, 12))
+    radius = np.zeros(n)
+    ar = np.ones(n)
+    hist = []
+    for p in range(n):
+        tok = lines[pos].split()
+        pos += 1
+        if len(tok) not in (14, 15, 16):
+            raise ValueError(f"{path}: particle line {pos} has {len(tok)} fields")
+        if int(tok[0]) != p:
+            raise ValueError(f"{path}: particle {p} carries ID {tok[0]}")
+        radius[p] = float(tok[1])
+        extra = len(tok) - 14
+        if extra == 2:
+            ar[p] = float(tok[3])
+        vals = [float(t) for t in tok[2 + extra:13 + extra]]
+        fields[p, :11] = vals
+        nc = int(tok[-1])
+        for _ in range(nc):
+            ct = lines[pos].split()
+            pos += 1
+            if len(ct) != 26:
+                raise ValueError(f"{path}: contact line {pos} has {len(ct)} fields")
+            v = [float(t) for t in ct[1:]]
+            # partner, shear overlap, normal, damping
+            hist.append((p, int(ct[0]), v[10], v[11], v[0], v[1], v[24]))
+    if aspect_ratio is not None:
+        ar = np.broadcast_to(np.asarray(aspect_ratio, np.float64), (n,)).copy()
+    k = KIND_ELLIPSE if kind == "ellipse" else KIND_CIRCLE
+    if k == KIND_CIRCLE:
+        ar = np.ones(n)
+    x, y, vx, vy, th, w, _vol, m, _im, inr, _iin = fields[:, :11].T
+    s = State(cfg, np.full(n, k, np.int32), np.arange(n, dtype=np.int32), np.zeros(n, np.int32),
+              radius, ar, m.copy(), inr.copy(), x.copy(), y.copy(), th.copy(), vx.copy(),
+              vy.copy(), w.copy(), np.zeros(n), np.zeros(n))
+    if hist:
+        h = np.array(hist, dtype=object)
+        s = s.with_history(h[:, 0].astype(np.int32), h[:, 1].astype(np.int32),
+                           h[:, 2].astype(np.float64), h[:, 3].astype(np.float64),
+                           h[:, 4].astype(np.float64), h[:, 5].astype(np.float64),
+                           h[:, 6].astype(np.float64))
+    return s

@@ -145,6 +145,7 @@ struct Table {
     double *xix, *xiy, *nx, *ny, *c;
     FRec* fr;
     double *fnx, *fny, *fsx, *fsy, *bx, *by, *px, *py;
+    double *b1x, *b1y, *b2x, *b2y;  // branch vectors from P1 and P2 to the point
     int* ccounter;                // kSeg contact-allocation counters
     int cseg_cap;                 // contacts per segment
 };
@@ -287,6 +288,8 @@ __device__ __forceinline__ void copy_contact(const Table& from, int a, const Tab
     to.fsx[b] = from.fsx[a]; to.fsy[b] = from.fsy[a];
     to.bx[b] = from.bx[a]; to.by[b] = from.by[a];
     to.px[b] = from.px[a]; to.py[b] = from.py[a];
+    to.b1x[b] = from.b1x[a]; to.b1y[b] = from.b1y[a];
+    to.b2x[b] = from.b2x[a]; to.b2y[b] = from.b2y[a];
 }
 
 // BaseParticle::isRattlerParticle (src/BaseParticle.cpp:153-156); a clump is
@@ -1142,6 +1145,8 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
         new_t.bx[ct] = xi - xj2;
         new_t.by[ct] = yi - yj2;
         new_t.px[ct] = g.px; new_t.py[ct] = g.py;
+        new_t.b1x[ct] = g.b1x; new_t.b1y[ct] = g.b1y;
+        new_t.b2x[ct] = g.b2x; new_t.b2y[ct] = g.b2y;
     }
 }
 
@@ -1840,6 +1845,7 @@ __global__ void k_load_contacts(Table t, int nc, const int* ci, const int* cj,
     t.fnx[k] = 0; t.fny[k] = 0; t.fsx[k] = 0; t.fsy[k] = 0;
     t.fr[k] = FRec{0.0, 0.0, 0.0, 0.0};
     t.bx[k] = 0; t.by[k] = 0; t.px[k] = 0; t.py[k] = 0;
+    t.b1x[k] = 0; t.b1y[k] = 0; t.b2x[k] = 0; t.b2y[k] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1974,6 +1980,37 @@ int segment_counts(xdem_engine* E, const Table& t, std::vector<int>& used) {
     return XDEM_OK;
 }
 
+// The contacts of table t that the sums see (reachable from the slots in
+// use), as indices into its contact list ordered by the external IDs
+// (P1, P2); li / lj hold the external IDs per contact-list entry.
+int contact_order(xdem_engine* E, const Table& t, std::vector<int>& li, std::vector<int>& lj,
+                  std::vector<int>& idx) {
+    int rc;
+    std::vector<int> seg_used(kSeg, 0);
+    if ((rc = segment_counts(E, t, seg_used))) return rc;
+    const size_t used = (size_t)t.cap, ccap = (size_t)t.cseg_cap * kSeg;
+    std::vector<int> link(used);
+    li.assign(ccap, 0);
+    lj.assign(ccap, 0);
+    HIPCHK(hipMemcpy(link.data(), t.link, sizeof(int) * used, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(li.data(), t.ci, sizeof(int) * ccap, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lj.data(), t.cj, sizeof(int) * ccap, hipMemcpyDeviceToHost));
+    idx.clear();
+    idx.reserve(used / 4);
+    for (size_t e = 0; e < used; ++e) {
+        if ((int)(e % t.seg_cap) >= seg_used[e / t.seg_cap]) continue;
+        const int c = link[e];
+        if (c < 0) continue;
+        li[c] = E->perm[li[c]];
+        lj[c] = E->perm[lj[c]];
+        idx.push_back(c);
+    }
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) {
+        return li[a] != li[b] ? li[a] < li[b] : lj[a] < lj[b];
+    });
+    return XDEM_OK;
+}
+
 int check_error(xdem_engine* E) {
     uint32_t bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, E->err_d, sizeof(bits), hipMemcpyDeviceToHost, E->stream));
@@ -2004,7 +2041,7 @@ int alloc_table(xdem_engine* E, Table& t, int cap, int n, const Table* share, in
         return rc;
     HIPCHK(hipMemset(t.ccounter, 0, sizeof(int) * kSeg * kCtrStride));
     double** d[] = {&t.xix, &t.xiy, &t.nx, &t.ny, &t.c, &t.fnx, &t.fny, &t.fsx,
-                    &t.fsy, &t.bx, &t.by, &t.px, &t.py};
+                    &t.fsy, &t.bx, &t.by, &t.px, &t.py, &t.b1x, &t.b1y, &t.b2x, &t.b2y};
     for (double** q : d)
         if ((rc = dalloc(E, q, ccap))) return rc;
     if ((rc = dalloc(E, &t.fr, ccap))) return rc;
@@ -2590,27 +2627,9 @@ int xdem_get_contacts(xdem_engine* E, int cap, int* nc, int* ci, int* cj, double
     if (rc) return rc;
     if (!nc) return fail(E, XDEM_E_INVALID, "nc is NULL");
     const Table& t = E->T[E->cur];
-    std::vector<int> seg_used(kSeg, 0);
-    if ((rc = segment_counts(E, t, seg_used))) return rc;
-    // contacts reachable from the slots in use (what the sums see)
-    const size_t used = (size_t)t.cap, ccap = (size_t)t.cseg_cap * kSeg;
-    std::vector<int> link(used), li(ccap), lj(ccap);
-    HIPCHK(hipMemcpy(link.data(), t.link, sizeof(int) * used, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(li.data(), t.ci, sizeof(int) * ccap, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(lj.data(), t.cj, sizeof(int) * ccap, hipMemcpyDeviceToHost));
-    std::vector<int> idx;
-    idx.reserve(used / 4);
-    for (size_t e = 0; e < used; ++e) {
-        if ((int)(e % t.seg_cap) >= seg_used[e / t.seg_cap]) continue;
-        const int c = link[e];
-        if (c < 0) continue;
-        li[c] = E->perm[li[c]];
-        lj[c] = E->perm[lj[c]];
-        idx.push_back(c);
-    }
-    std::sort(idx.begin(), idx.end(), [&](int a, int b) {
-        return li[a] != li[b] ? li[a] < li[b] : lj[a] < lj[b];
-    });
+    std::vector<int> li, lj, idx;
+    if ((rc = contact_order(E, t, li, lj, idx))) return rc;
+    const size_t ccap = (size_t)t.cseg_cap * kSeg;
     *nc = (int)idx.size();
     const int m = std::min<int>(cap, (int)idx.size());
     std::vector<double> buf(ccap);
@@ -2637,6 +2656,31 @@ int xdem_get_contacts(xdem_engine* E, int cap, int* nc, int* ci, int* cj, double
             if (torque_j) torque_j[k] = fr[idx[k]].t2;
         }
     }
+    return XDEM_OK;
+}
+
+int xdem_get_contact_branches(xdem_engine* E, int cap, int* nc, double* branch_x,
+                              double* branch_y, double* b1_x, double* b1_y, double* b2_x,
+                              double* b2_y) {
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    if (!nc) return fail(E, XDEM_E_INVALID, "nc is NULL");
+    const Table& t = E->T[E->cur];
+    std::vector<int> li, lj, idx;
+    if ((rc = contact_order(E, t, li, lj, idx))) return rc;
+    const size_t ccap = (size_t)t.cseg_cap * kSeg;
+    *nc = (int)idx.size();
+    const int m = std::min<int>(cap, (int)idx.size());
+    std::vector<double> buf(ccap);
+    auto fetch = [&](double* dst, const double* src) -> int {
+        if (!dst) return XDEM_OK;
+        HIPCHK(hipMemcpy(buf.data(), src, sizeof(double) * ccap, hipMemcpyDeviceToHost));
+        for (int k = 0; k < m; ++k) dst[k] = buf[idx[k]];
+        return XDEM_OK;
+    };
+    if ((rc = fetch(branch_x, t.bx)) || (rc = fetch(branch_y, t.by)) || (rc = fetch(b1_x, t.b1x)) ||
+        (rc = fetch(b1_y, t.b1y)) || (rc = fetch(b2_x, t.b2x)) || (rc = fetch(b2_y, t.b2y)))
+        return rc;
     return XDEM_OK;
 }
 

@@ -25,7 +25,8 @@ ERRORS = {-1: "XDEM_E_INVALID", -2: "XDEM_E_CAPACITY", -3: "XDEM_E_UNSUPPORTED_P
 # exported entry points of include/xdem.h
 SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles",
            "xdem_set_contacts", "xdem_step", "xdem_sync", "xdem_get_particles",
-           "xdem_get_forces", "xdem_get_contacts", "xdem_get_stress", "xdem_get_boundary",
+           "xdem_get_forces", "xdem_get_contacts", "xdem_get_contact_branches",
+           "xdem_get_stress", "xdem_get_boundary",
            "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
            "xdem_set_control", "xdem_get_box", "xdem_get_observables",
            "xdem_history_open", "xdem_history_write", "xdem_history_close",
@@ -80,6 +81,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_get_particles.argtypes = [vp] + [_D] * 6
     L.xdem_get_forces.argtypes = [vp, _D, _D, _D, _I]
     L.xdem_get_contacts.argtypes = [vp, C.c_int, C.POINTER(C.c_int)] + [vp] * 15
+    L.xdem_get_contact_branches.argtypes = [vp, C.c_int, C.POINTER(C.c_int)] + [vp] * 6
     L.xdem_get_stress.argtypes = [vp, _D]
     L.xdem_get_boundary.argtypes = [vp, _D]
     L.xdem_get_counters.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 7
@@ -220,6 +222,19 @@ class Engine:
         d = dict(zip(names, arrs))
         d["i"], d["j"] = ci, cj
         return d
+
+    def contact_branches(self):
+        """Branch vectors of the contacts() entries, same order
+        (xdem_get_contact_branches)."""
+        nc = C.c_int(0)
+        self._check(self.L.xdem_get_contact_branches(self.h, 0, C.byref(nc), *([None] * 6)))
+        m = nc.value
+        names = ("branch_x", "branch_y", "b1_x", "b1_y", "b2_x", "b2_y")
+        arrs = [np.zeros(m) for _ in names]
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.L.xdem_get_contact_branches(self.h, m, C.byref(nc),
+                                                     *[ptr(a) for a in arrs]))
+        return dict(zip(names, arrs))
 
     def stress(self):
         out = np.zeros(4 * len(self.states))

@@ -73,6 +73,8 @@ struct Pack {
     double cell, vcell, rmax;     // reference-grid cell; smallest list-grid cell
     int pnx, pny;                 // reference grid of the last step (the cells in P.cell)
     double sigma;                 // product of the shrink factors since the rebuild
+    double risk_r;                // min(lenx, leny) - skin: a contact with
+                                  // r_i + r_j >= risk_r may be carried forward
 };
 
 // Per-particle ellipse quantities.  Fixed: b = a/AR, e^2, b^2, b/a - 1 and
@@ -92,6 +94,16 @@ struct Frame {
 // P1 and the torques b1 x F, b2 x (-F) (src/ContactInformation.h:62-68).
 struct alignas(32) FRec {
     double fx, fy, t1, t2;
+};
+
+// Detail fields of a contact: normal and tangential force, branch vector
+// (P1's centre minus P2's image centre), contact point, and the branch
+// vectors from P1 and P2 to the point (src/ContactInformation.h:15-29).  One
+// 96-byte record per contact, so that a contact whose details are written
+// alone fills most of a cache line instead of one 8-byte word in each of
+// twelve arrays.
+struct alignas(16) DRec {
+    double fnx, fny, fsx, fsy, bx, by, px, py, b1x, b1y, b2x, b2y;
 };
 
 struct Parts {
@@ -144,17 +156,23 @@ struct Table {
     int *ci, *cj;                 // P1, P2 (internal indices)
     double *xix, *xiy, *nx, *ny, *c;
     FRec* fr;
-    double *fnx, *fny, *fsx, *fsy, *bx, *by, *px, *py;
-    double *b1x, *b1y, *b2x, *b2y;  // branch vectors from P1 and P2 to the point
+    DRec* det;                    // written on the steps that need it (k_contact)
     int* ccounter;                // kSeg contact-allocation counters
     int cseg_cap;                 // contacts per segment
 };
 
 // Pairs that passed the bounding-circle test this step: x = slot | (image
-// list << kLstShift), y = slot of the pair's history in the previous table
-// or -1, z = P1, w = P2 (so the contact kernel starts its particle loads
-// without first reading the pair list).  Segmented like the table.
+// list << kLstShift), y = index of the pair's history in the previous
+// contact list + 1 (0: none) with bit 31 set when the contact's detail
+// record must be written on every step (contact_item), z = P1, w = P2 (so
+// the contact kernel starts its particle loads without first reading the
+// pair list).  Segmented like the table.
 constexpr int kLstShift = 30;
+__host__ __device__ inline int item_y(int found, bool risky) {
+    return (int)((unsigned)(found + 1) | (risky ? 0x80000000u : 0u));
+}
+__host__ __device__ inline int item_found(int y) { return (int)((unsigned)y & 0x7fffffffu) - 1; }
+__host__ __device__ inline bool item_risky(int y) { return y < 0; }
 // In the learned-guess mode each item also carries the pair's features
 // (BF16) and the network's guess.
 struct WorkList {
@@ -248,19 +266,20 @@ __device__ __forceinline__ int xcd_segment() {
     return (int)((long long)xcd_block() * kSeg / (long long)gridDim.x);
 }
 
-// Contact-list index for one contact (rare paths: one atomic each); -1 and
-// an error bit when the segment is full.
-__device__ __forceinline__ int alloc_contact_one(const Table& t, uint32_t* err) {
+// Index in a segmented list (kSeg segments of seg_cap entries, one counter
+// each) for one entry (rare paths: one atomic each); -1 and an error bit
+// when the segment is full.
+__device__ __forceinline__ int alloc_one(int* counter, int seg_cap, uint32_t* err) {
     const int seg = xcd_segment();
-    const int k = atomicAdd(t.ccounter + seg * kCtrStride, 1);
-    if (k >= t.cseg_cap) { raise_err(err, ERR_PAIR_OVERFLOW); return -1; }
-    return seg * t.cseg_cap + k;
+    const int k = atomicAdd(counter + seg * kCtrStride, 1);
+    if (k >= seg_cap) { raise_err(err, ERR_PAIR_OVERFLOW); return -1; }
+    return seg * seg_cap + k;
 }
 
-// Contact-list indices for the lanes of a wavefront with `want` set:
+// Indices in a segmented list for the lanes of a wavefront with `want` set:
 // consecutive indices in lane order, one atomic per wavefront.  Every
 // active lane of the wavefront must call it (converged control flow).
-__device__ __forceinline__ int alloc_contact_wave(const Table& t, bool want, uint32_t* err) {
+__device__ __forceinline__ int alloc_wave(int* counter, int seg_cap, bool want, uint32_t* err) {
     const unsigned long long m = __ballot(want ? 1 : 0);
     if (m == 0ULL) return -1;
     const int lane = threadIdx.x & 63;
@@ -268,14 +287,14 @@ __device__ __forceinline__ int alloc_contact_wave(const Table& t, bool want, uin
     const int cnt = __popcll(m);
     const int seg = xcd_segment();
     int base = 0;
-    if (lane == leader) base = atomicAdd(t.ccounter + seg * kCtrStride, cnt);
+    if (lane == leader) base = atomicAdd(counter + seg * kCtrStride, cnt);
     base = __shfl(base, leader, 64);
-    if (base + cnt > t.cseg_cap) {
+    if (base + cnt > seg_cap) {
         if (lane == leader) raise_err(err, ERR_PAIR_OVERFLOW);
         return -1;
     }
     if (!want) return -1;
-    return seg * t.cseg_cap + base + __popcll(m & ((1ULL << lane) - 1ULL));
+    return seg * seg_cap + base + __popcll(m & ((1ULL << lane) - 1ULL));
 }
 
 // Copy one contact record between contact lists.
@@ -284,12 +303,7 @@ __device__ __forceinline__ void copy_contact(const Table& from, int a, const Tab
     to.xix[b] = from.xix[a]; to.xiy[b] = from.xiy[a];
     to.nx[b] = from.nx[a]; to.ny[b] = from.ny[a]; to.c[b] = from.c[a];
     to.fr[b] = from.fr[a];
-    to.fnx[b] = from.fnx[a]; to.fny[b] = from.fny[a];
-    to.fsx[b] = from.fsx[a]; to.fsy[b] = from.fsy[a];
-    to.bx[b] = from.bx[a]; to.by[b] = from.by[a];
-    to.px[b] = from.px[a]; to.py[b] = from.py[a];
-    to.b1x[b] = from.b1x[a]; to.b1y[b] = from.b1y[a];
-    to.b2x[b] = from.b2x[a]; to.b2y[b] = from.b2y[a];
+    to.det[b] = from.det[a];
 }
 
 // BaseParticle::isRattlerParticle (src/BaseParticle.cpp:153-156); a clump is
@@ -734,7 +748,7 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
     for (int w0 = wbeg; w0 < wend; w0 += kBlock) {
         const int w = w0 + threadIdx.x;
         bool push = false;
-        int4 item = make_int4(0, -1, 0, 0);
+        int4 item = make_int4(0, 0, 0, 0);
         float feat[kMlpIn];
         if (PREC == kNN)
             for (int k = 0; k < kMlpIn; ++k) feat[k] = 0.f;
@@ -767,13 +781,14 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
                 } else {
                     lst = cell_list(P, K, i, j);
                 }
+                const double rsum = P.r[i] + P.r[j];
                 if (lst >= 0 && shapes) {
                     const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
                     double ox, oy, ovx;
                     image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
                     if (PREC == kMixed) {
                         const float dxf = (float)((xj + ox) - xi), dyf = (float)((yj + oy) - yi);
-                        const float rs = (float)(P.r[i] + P.r[j]) * kCullMargin;
+                        const float rs = (float)rsum * kCullMargin;
                         push = rs >= sqrtf(dxf * dxf + dyf * dyf);
                         n_coarse += (ki == KIND_CIRCLE || push) ? 1 : 0;
                         if (push && ki == KIND_ELLIPSE && !K.strict)
@@ -782,11 +797,11 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
                                                         kPwMarginF32);
                     } else if (ki == KIND_CIRCLE) {
                         const double bvx = (xj + ox) - xi, bvy = (yj + oy) - yi;
-                        push = !((P.r[i] + P.r[j]) - sqrt(bvx * bvx + bvy * bvy) < 0.0);
+                        push = !(rsum - sqrt(bvx * bvx + bvy * bvy) < 0.0);
                         ++n_coarse;
                     } else {
                         const double qx = xi - (xj + ox), qy = yi - (yj + oy);
-                        push = (P.r[i] + P.r[j]) > sqrt(qx * qx + qy * qy);
+                        push = rsum > sqrt(qx * qx + qy * qy);
                         n_coarse += push ? 1 : 0;
                         if (push && !K.strict)
                             push = !pw_separated<double>(-qx, -qy, P.r[i], F.b[i], F.kx[i], F.ky[i],
@@ -800,9 +815,13 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
                     }
                 }
                 if (push) {
-                    item = make_int4((int)((unsigned)e | ((unsigned)lst << kLstShift)), found, i, j);
+                    // a contact that the next step could carry forward unevaluated
+                    // (contact_item)
+                    const bool risky = rsum >= K.risk_r;
+                    item = make_int4((int)((unsigned)e | ((unsigned)lst << kLstShift)),
+                                     item_y(found, risky), i, j);
                 } else if (found >= 0 && lst < 0) {     // not a candidate: carried forward
-                    const int c = alloc_contact_one(new_t, err);
+                    const int c = alloc_one(new_t.ccounter, new_t.cseg_cap, err);
                     if (c >= 0) copy_contact(old_t, found, new_t, c);
                     new_t.link[e] = c;
                 } else {
@@ -997,7 +1016,7 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
         if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); hit = 0; }
         if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); hit = 0; }
     }
-    const int ct = alloc_contact_wave(new_t, hit != 0, err);
+    const int ct = alloc_wave(new_t.ccounter, new_t.cseg_cap, hit != 0, err);
     new_t.link[e] = ct;
     if (ct < 0) return;
     const bool le = K.world == WORLD_LE;
@@ -1138,15 +1157,15 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
     // displacements and the Lees-Edwards shift change by it), and two centres
     // closer than a cell length lie in adjacent cells, so the pair stays a
     // candidate and cannot be carried forward unless
-    // r_i + r_j + skin >= the shorter cell length.
-    if (record || P.r[i] + P.r[j] + K.skin >= fmin(K.lenx, K.leny)) {
-        new_t.fnx[ct] = fnx; new_t.fny[ct] = fny;
-        new_t.fsx[ct] = fsx; new_t.fsy[ct] = fsy;
-        new_t.bx[ct] = xi - xj2;
-        new_t.by[ct] = yi - yj2;
-        new_t.px[ct] = g.px; new_t.py[ct] = g.py;
-        new_t.b1x[ct] = g.b1x; new_t.b1y[ct] = g.b1y;
-        new_t.b2x[ct] = g.b2x; new_t.b2y[ct] = g.b2y;
+    // r_i + r_j + skin >= the shorter cell length (k_coarse marks such
+    // items).  One 96-byte record, so a lone writer fills most of a line.
+    if (record) {
+        DRec d;
+        d.fnx = fnx; d.fny = fny; d.fsx = fsx; d.fsy = fsy;
+        d.bx = xi - xj2; d.by = yi - yj2;
+        d.px = g.px; d.py = g.py;
+        d.b1x = g.b1x; d.b1y = g.b1y; d.b2x = g.b2x; d.b2y = g.b2y;
+        new_t.det[ct] = d;
     }
 }
 
@@ -1179,7 +1198,8 @@ k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table n
         const int i = it.z, j = it.w;
         const float guess = PREC == kNN ? W.guess[at] : 0.0f;
         const Pack K = packs[P.pack[i]];
-        contact_item<PREC>(P, F, K, old_t, new_t, e, lst, it.y, i, j, guess, record, err, n);
+        contact_item<PREC>(P, F, K, old_t, new_t, e, lst, item_found(it.y), i, j, guess,
+                           record || item_risky(it.y), err, n);
     }
     Counters* cs = counter_slot(cnt);
     count_add(&cs->newton, n.newton);
@@ -1486,8 +1506,9 @@ k_stress(Parts P, Table t, double* out) {
                 const int c = t.link[e];
                 if (c < 0) continue;
                 const double fx = t.fr[c].fx, fy = t.fr[c].fy;
-                const double bx = clump ? t.bx[c] + ox : t.bx[c];
-                const double by = clump ? t.by[c] + oy : t.by[c];
+                const DRec& d = t.det[c];
+                const double bx = clump ? d.bx + ox : d.bx;
+                const double by = clump ? d.by + oy : d.by;
                 sxx += fx * bx; sxy += fx * by; syx += fy * bx; syy += fy * by;
             }
         }
@@ -1549,6 +1570,7 @@ __device__ bool regrid(Pack& K, uint32_t* err) {
         K.skin = fmax(0.0, sk);
         changed = true;
     }
+    K.risk_r = fmin(K.lenx, K.leny) - K.skin;
     return changed;
 }
 
@@ -1739,7 +1761,7 @@ k_obs_particles(Parts P, const Pack* __restrict__ packs, Table t, const int* __r
     for (int e = t.row_start[p]; e < t.row_start[p] + t.row_cnt[p]; ++e) {
         const int c = t.link[e];
         if (c < 0) continue;
-        const double ox = t.fnx[c] / K.kn, oy = t.fny[c] / K.kn;
+        const double ox = t.det[c].fnx / K.kn, oy = t.det[c].fny / K.kn;
         el += ox * ox + oy * oy;
         ++nc;
     }
@@ -1842,10 +1864,8 @@ __global__ void k_load_contacts(Table t, int nc, const int* ci, const int* cj,
     t.link[e] = k;
     t.ci[k] = ci[k]; t.cj[k] = cj[k];
     t.xix[k] = xix[k]; t.xiy[k] = xiy[k]; t.nx[k] = nx[k]; t.ny[k] = ny[k]; t.c[k] = c[k];
-    t.fnx[k] = 0; t.fny[k] = 0; t.fsx[k] = 0; t.fsy[k] = 0;
     t.fr[k] = FRec{0.0, 0.0, 0.0, 0.0};
-    t.bx[k] = 0; t.by[k] = 0; t.px[k] = 0; t.py[k] = 0;
-    t.b1x[k] = 0; t.b1y[k] = 0; t.b2x[k] = 0; t.b2y[k] = 0;
+    t.det[k] = DRec{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 }
 
 // ---------------------------------------------------------------------------
@@ -2011,6 +2031,17 @@ int contact_order(xdem_engine* E, const Table& t, std::vector<int>& li, std::vec
     return XDEM_OK;
 }
 
+// Detail records of the contacts idx[0..m) of table t.
+int fetch_details(xdem_engine* E, const Table& t, const std::vector<int>& idx, int m,
+                  std::vector<DRec>& out) {
+    const size_t ccap = (size_t)t.cseg_cap * kSeg;
+    std::vector<DRec> det(ccap);
+    HIPCHK(hipMemcpy(det.data(), t.det, sizeof(DRec) * ccap, hipMemcpyDeviceToHost));
+    out.resize(m);
+    for (int k = 0; k < m; ++k) out[k] = det[idx[k]];
+    return XDEM_OK;
+}
+
 int check_error(xdem_engine* E) {
     uint32_t bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, E->err_d, sizeof(bits), hipMemcpyDeviceToHost, E->stream));
@@ -2040,11 +2071,10 @@ int alloc_table(xdem_engine* E, Table& t, int cap, int n, const Table* share, in
         (rc = dalloc(E, &t.ccounter, kSeg * kCtrStride)))
         return rc;
     HIPCHK(hipMemset(t.ccounter, 0, sizeof(int) * kSeg * kCtrStride));
-    double** d[] = {&t.xix, &t.xiy, &t.nx, &t.ny, &t.c, &t.fnx, &t.fny, &t.fsx,
-                    &t.fsy, &t.bx, &t.by, &t.px, &t.py, &t.b1x, &t.b1y, &t.b2x, &t.b2y};
+    double** d[] = {&t.xix, &t.xiy, &t.nx, &t.ny, &t.c};
     for (double** q : d)
         if ((rc = dalloc(E, q, ccap))) return rc;
-    if ((rc = dalloc(E, &t.fr, ccap))) return rc;
+    if ((rc = dalloc(E, &t.fr, ccap)) || (rc = dalloc(E, &t.det, ccap))) return rc;
     if ((rc = dalloc(E, &t.row_start, n)) || (rc = dalloc(E, &t.row_cnt, n)) ||
         (rc = dalloc(E, &t.counter, kSeg * kCtrStride)))
         return rc;
@@ -2220,6 +2250,7 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
         K.vnx_alloc = K.vnx; K.vny_alloc = K.vny;
         K.pnx = K.nx; K.pny = K.ny;
         K.vcell = 2.0 * rmax + K.skin;      // after the skin fits the grid
+        K.risk_r = std::min(K.lenx, K.leny) - K.skin;
         K.bin_base = bins_total;
         bins_total += (K.vny + 2) * K.vnx;
         if (amin < 1e300) max_ratio = std::max(max_ratio, K.vlenx * K.vleny / amin);
@@ -2643,11 +2674,22 @@ int xdem_get_contacts(xdem_engine* E, int cap, int* nc, int* ci, int* cj, double
         if (ci) ci[k] = li[idx[k]];
         if (cj) cj[k] = lj[idx[k]];
     }
-    if ((rc = fetch(fn_x, t.fnx)) || (rc = fetch(fn_y, t.fny)) || (rc = fetch(fs_x, t.fsx)) ||
-        (rc = fetch(fs_y, t.fsy)) || (rc = fetch(point_x, t.px)) || (rc = fetch(point_y, t.py)) ||
-        (rc = fetch(xi_x, t.xix)) || (rc = fetch(xi_y, t.xiy)) || (rc = fetch(n_x, t.nx)) ||
+    if ((rc = fetch(xi_x, t.xix)) || (rc = fetch(xi_y, t.xiy)) || (rc = fetch(n_x, t.nx)) ||
         (rc = fetch(n_y, t.ny)) || (rc = fetch(damping, t.c)))
         return rc;
+    if (fn_x || fn_y || fs_x || fs_y || point_x || point_y) {
+        std::vector<DRec> det;
+        if ((rc = fetch_details(E, t, idx, m, det))) return rc;
+        for (int k = 0; k < m; ++k) {
+            const DRec& d = det[k];
+            if (fn_x) fn_x[k] = d.fnx;
+            if (fn_y) fn_y[k] = d.fny;
+            if (fs_x) fs_x[k] = d.fsx;
+            if (fs_y) fs_y[k] = d.fsy;
+            if (point_x) point_x[k] = d.px;
+            if (point_y) point_y[k] = d.py;
+        }
+    }
     if (torque_i || torque_j) {
         std::vector<FRec> fr(ccap);
         HIPCHK(hipMemcpy(fr.data(), t.fr, sizeof(FRec) * ccap, hipMemcpyDeviceToHost));
@@ -2671,16 +2713,17 @@ int xdem_get_contact_branches(xdem_engine* E, int cap, int* nc, double* branch_x
     const size_t ccap = (size_t)t.cseg_cap * kSeg;
     *nc = (int)idx.size();
     const int m = std::min<int>(cap, (int)idx.size());
-    std::vector<double> buf(ccap);
-    auto fetch = [&](double* dst, const double* src) -> int {
-        if (!dst) return XDEM_OK;
-        HIPCHK(hipMemcpy(buf.data(), src, sizeof(double) * ccap, hipMemcpyDeviceToHost));
-        for (int k = 0; k < m; ++k) dst[k] = buf[idx[k]];
-        return XDEM_OK;
-    };
-    if ((rc = fetch(branch_x, t.bx)) || (rc = fetch(branch_y, t.by)) || (rc = fetch(b1_x, t.b1x)) ||
-        (rc = fetch(b1_y, t.b1y)) || (rc = fetch(b2_x, t.b2x)) || (rc = fetch(b2_y, t.b2y)))
-        return rc;
+    std::vector<DRec> det;
+    if ((rc = fetch_details(E, t, idx, m, det))) return rc;
+    for (int k = 0; k < m; ++k) {
+        const DRec& d = det[k];
+        if (branch_x) branch_x[k] = d.bx;
+        if (branch_y) branch_y[k] = d.by;
+        if (b1_x) b1_x[k] = d.b1x;
+        if (b1_y) b1_y[k] = d.b1y;
+        if (b2_x) b2_x[k] = d.b2x;
+        if (b2_y) b2_y[k] = d.b2y;
+    }
     return XDEM_OK;
 }
 

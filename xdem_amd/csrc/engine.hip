@@ -79,16 +79,25 @@ struct Pack {
 
 // Per-particle ellipse quantities.  Fixed: b = a/AR, e^2, b^2, b/a - 1 and
 // a/b - 1.  Per step (from theta after the wrap): the orientation k and,
-// for the particle's role as P1, the matrices T = (I + kk (b/a-1))/b, T^-1,
-// A = (I - e^2 kk)/b^2 and I + kk (a/b-1); each is symmetric, so three
-// entries.  These are the per-particle subexpressions of
-// src/EllipticParticle.cpp:49-73 and :209-216, evaluated once per step by the
-// same operations, so every pair reads the bits it would have computed.
+// for the particle's role as P1, the matrices T = (I + kk (b/a-1))/b, T^-1
+// and A = (I - e^2 kk)/b^2; each is symmetric, so three entries.  These are
+// the per-particle subexpressions of src/EllipticParticle.cpp:49-73 and
+// :209-216, evaluated once per step by the same operations, so every pair
+// reads the bits it would have computed.  I + kk (a/b-1) costs two
+// operations per entry and is formed by the pair kernels (frame_m).
 struct Frame {
     const double *b, *e2, *bsq, *s1, *eta;
     double *kx, *ky;
-    double *txx, *txy, *tyy, *ixx, *ixy, *iyy, *axx, *axy, *ayy, *mxx, *mxy, *myy;
+    double *txx, *txy, *tyy, *ixx, *ixy, *iyy, *axx, *axy, *ayy;
 };
+
+// I + k1k1 eta of P1 (src/EllipticParticle.cpp:57-60), from its orientation
+// and eta = a/b - 1.
+__device__ __forceinline__ void frame_m(double kx, double ky, double eta, double& mxx,
+                                        double& mxy, double& myy) {
+    const double kxx = kx * kx, kxy = kx * ky, kyy = ky * ky;
+    mxx = 1.0 + kxx * eta; mxy = 0.0 + kxy * eta; myy = 1.0 + kyy * eta;
+}
 
 // Force record of a contact, read by the per-particle sums: F = Fn + Fs on
 // P1 and the torques b1 x F, b2 x (-F) (src/ContactInformation.h:62-68).
@@ -372,12 +381,12 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
         }
     }
     if (P.kind[p] == KIND_ELLIPSE) {
-        // src/EllipticParticle.cpp:53-73 (P1: k1, T, I + k1k1 eta) and
-        // :211-216 (T^-1, A1); P2's I - k2k2 e2^2 is formed per pair from k
+        // src/EllipticParticle.cpp:53-73 (P1: k1, T) and :211-216 (T^-1, A1);
+        // I + k1k1 eta and P2's I - k2k2 e2^2 are formed per pair from k
         double s, c;
         sincos(P.th[p], &s, &c);
         const double kxx = c * c, kxy = c * s, kyy = s * s;
-        const double b = F.b[p], s1 = F.s1[p], eta = F.eta[p], e2 = F.e2[p], bsq = F.bsq[p];
+        const double b = F.b[p], s1 = F.s1[p], e2 = F.e2[p], bsq = F.bsq[p];
         const double txx = (1.0 + kxx * s1) / b, txy = (0.0 + kxy * s1) / b;
         const double tyy = (1.0 + kyy * s1) / b;
         const double tdet = txx * tyy - txy * txy;
@@ -387,7 +396,6 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
         F.axx[p] = (1.0 - kxx * e2) / bsq;
         F.axy[p] = (0.0 - kxy * e2) / bsq;
         F.ayy[p] = (1.0 - kyy * e2) / bsq;
-        F.mxx[p] = 1.0 + kxx * eta; F.mxy[p] = 0.0 + kxy * eta; F.myy[p] = 1.0 + kyy * eta;
     }
     const Pack K = packs[P.pack[p]];
     double x = P.x[p], y = P.y[p];
@@ -723,7 +731,9 @@ __device__ __forceinline__ bool pw_separated(T dx, T dy, double a1, double b1, d
     return q > (T)2 * ((T)1 + margin) * det;
 }
 
-template <int PREC>
+// ONE: the engine holds a single packing, so every lane reads the same
+// packing record (scalar loads instead of one vector load per field).
+template <int PREC, bool ONE>
 __global__ void __launch_bounds__(kBlock)
 k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t, WorkList W,
          const int* __restrict__ rebuild, uint32_t* err, Counters* cnt) {
@@ -759,7 +769,7 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
             const int code = src_j[e];
             const int j = code & ~kExtraBit;
             if (i >= 0 && i < P.n && j >= 0 && j < P.n) {
-                const Pack K = packs[P.pack[i]];
+                const Pack K = packs[ONE ? 0 : P.pack[i]];
                 // history of (i, j) as an index into the previous contact
                 // list: the same slot when the list was not rebuilt
                 int found = -1;
@@ -985,7 +995,7 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
         ZpmFrame f;
         f.kx = F.kx[i]; f.ky = F.ky[i];
         f.txx = F.txx[i]; f.txy = F.txy[i]; f.tyy = F.tyy[i];
-        f.mxx = F.mxx[i]; f.mxy = F.mxy[i]; f.myy = F.myy[i];
+        frame_m(f.kx, f.ky, F.eta[i], f.mxx, f.mxy, f.myy);
         f.b = F.b[i]; f.e2 = F.e2[i]; f.bsq = F.bsq[i];
         const double k2x = F.kx[j], k2y = F.ky[j], e2s = F.e2[j], b2s = F.bsq[j];
         // attempt 0: the reference's arithmetic; attempt 1 only when that
@@ -1170,7 +1180,7 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
 }
 
 // One lane per work item.
-template <int PREC>
+template <int PREC, bool ONE>
 __global__ void __launch_bounds__(kBlock)
 k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t,
           WorkList W, const int* __restrict__ record_flag, uint32_t* err, Counters* cnt) {
@@ -1197,7 +1207,7 @@ k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table n
         const int lst = (int)(ux >> kLstShift);
         const int i = it.z, j = it.w;
         const float guess = PREC == kNN ? W.guess[at] : 0.0f;
-        const Pack K = packs[P.pack[i]];
+        const Pack K = packs[ONE ? 0 : P.pack[i]];
         contact_item<PREC>(P, F, K, old_t, new_t, e, lst, item_found(it.y), i, j, guess,
                            record || item_risky(it.y), err, n);
     }
@@ -1284,7 +1294,7 @@ k_log(Parts P, Frame F, const Pack* __restrict__ packs, Table new_t, WorkList W,
                 ZpmFrame f;
                 f.kx = F.kx[i]; f.ky = F.ky[i];
                 f.txx = F.txx[i]; f.txy = F.txy[i]; f.tyy = F.tyy[i];
-                f.mxx = F.mxx[i]; f.mxy = F.mxy[i]; f.myy = F.myy[i];
+                frame_m(f.kx, f.ky, F.eta[i], f.mxx, f.mxy, f.myy);
                 f.b = F.b[i]; f.e2 = F.e2[i]; f.bsq = F.bsq[i];
                 ZpmSetup S;
                 if (zpm_setup_frame(f, F.kx[j], F.ky[j], F.e2[j], F.bsq[j], dx, dy, S)) {
@@ -2341,19 +2351,18 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     HIPCHK(hipMemcpy(kind_d, k8.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(flags_d, f8.data(), n, hipMemcpyHostToDevice));
     {
-        // ellipse frame: 5 fixed and 14 per-step arrays of n doubles
+        // ellipse frame: 5 fixed and 11 per-step arrays of n doubles
         double* fb = nullptr;
-        if ((rc = dalloc(E, &fb, (size_t)19 * n))) return rc;
-        HIPCHK(hipMemset(fb, 0, sizeof(double) * 19 * (size_t)n));
-        double* q[19];
-        for (int k = 0; k < 19; ++k) q[k] = fb + (size_t)k * n;
+        if ((rc = dalloc(E, &fb, (size_t)16 * n))) return rc;
+        HIPCHK(hipMemset(fb, 0, sizeof(double) * 16 * (size_t)n));
+        double* q[16];
+        for (int k = 0; k < 16; ++k) q[k] = fb + (size_t)k * n;
         Frame& F = E->F;
         F.b = q[0]; F.e2 = q[1]; F.bsq = q[2]; F.s1 = q[3]; F.eta = q[4];
         F.kx = q[5]; F.ky = q[6];
         F.txx = q[7]; F.txy = q[8]; F.tyy = q[9];
         F.ixx = q[10]; F.ixy = q[11]; F.iyy = q[12];
         F.axx = q[13]; F.axy = q[14]; F.ayy = q[15];
-        F.mxx = q[16]; F.mxy = q[17]; F.myy = q[18];
         hipLaunchKernelGGL(k_shape, dim3(grid_of(n)), dim3(kBlock), 0, E->stream, n, r_d, ar_d,
                            kind_d, q[0], q[1], q[2], q[3], q[4]);
         HIPCHK(hipGetLastError());
@@ -2499,7 +2508,7 @@ int xdem_set_contacts(xdem_engine* E, int nc, const int* ci, const int* cj,
 namespace {
 
 // Enqueue one step of every packing on the engine's stream.
-template <int PREC>
+template <int PREC, bool ONE>
 int launch_step_t(xdem_engine* E) {
     const int gn = grid_of(E->n);
     Table& old_t = E->T[E->cur];
@@ -2522,13 +2531,13 @@ int launch_step_t(xdem_engine* E) {
         e0 = E->ev_pool[E->ev_used++];
         e1 = E->ev_pool[E->ev_used++];
     }
-    hipLaunchKernelGGL(k_coarse<PREC>, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
+    hipLaunchKernelGGL((k_coarse<PREC, ONE>), dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
                        E->F, E->packs_d, old_t, new_t, E->W, rb, E->err_d, E->cnt_d);
     if (PREC == kNN)
         hipLaunchKernelGGL(k_guess, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->W,
                            E->mlp_pk, E->err_d);
     if (E->timing) HIPCHK(hipEventRecord(e0, E->stream));
-    hipLaunchKernelGGL(k_contact<PREC>, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
+    hipLaunchKernelGGL((k_contact<PREC, ONE>), dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
                        E->F, E->packs_d, old_t, new_t, E->W, E->record_d, E->err_d, E->cnt_d);
     if (E->timing) HIPCHK(hipEventRecord(e1, E->stream));
     if (E->logging)
@@ -2555,9 +2564,10 @@ int launch_step_t(xdem_engine* E) {
 }
 
 int launch_step(xdem_engine* E) {
-    if (E->prec == kMixed) return launch_step_t<kMixed>(E);
-    if (E->prec == kNN) return launch_step_t<kNN>(E);
-    return launch_step_t<kFP64>(E);
+    const bool one = E->npacks == 1;
+    if (E->prec == kMixed) return one ? launch_step_t<kMixed, true>(E) : launch_step_t<kMixed, false>(E);
+    if (E->prec == kNN) return one ? launch_step_t<kNN, true>(E) : launch_step_t<kNN, false>(E);
+    return one ? launch_step_t<kFP64, true>(E) : launch_step_t<kFP64, false>(E);
 }
 
 // Capture two consecutive steps (starting at the current parity) as a graph.

@@ -1,7 +1,7 @@
 // engine.hip -- MI355X (gfx950) DEM step engine behind the C-ABI of
 // include/xdem.h.
 //
-// One step of every packing is seven launches on the engine's stream:
+// One step of every packing is six launches on the engine's stream:
 //   k_bin         wrap positions (modifyParticlePosition), cell index, the
 //                 per-particle ellipse frame of this step; on a list-rebuild
 //                 step also fills fixed-capacity cell bins and the two ghost
@@ -17,13 +17,13 @@
 //   k_contact     one lane per surviving pair: ZPM / circle geometry,
 //                 history, contact model
 //                 (BaseParticle::calculateContactForceAndTorque)
+//   k_integrate   per-particle force/torque sums in a fixed order
+//                 (collectForceAndTorque), then background-velocity removal,
+//                 integrator, clumps, LE shift (takeTimeIntegral,
+//                 updateLeesEdwardBoundary)
 // The pair list is rebuilt when 2 x (largest displacement of a particle
 // since the last rebuild) + (change of the LE shift) reaches the skin, so
 // every pair that can pass the bounding-circle test is listed.
-//   k_gather      per-particle force/torque sums in a fixed order
-//                 (collectForceAndTorque)
-//   k_integrate   background-velocity removal, integrator, clumps, LE shift
-//                 (takeTimeIntegral, updateLeesEdwardBoundary)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -357,7 +357,7 @@ __device__ __forceinline__ void image_offset(const Pack& K, int img, double xi, 
 // particle (src/PeriodicBoundaryWorld.cpp:32-33).  On a rebuild step the
 // cell bins and the two ghost rows are filled; otherwise the pair list
 // (rows and segment counters) carries over to the new table.
-template <int PREC>
+template <int PREC, bool ONE>
 __global__ void __launch_bounds__(kBlock)
 k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
       WorkList W, const int* __restrict__ rebuild, int* rebuild_next, uint32_t* err) {
@@ -373,8 +373,8 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
     {
         // scaleVelocityOfRattlers (src/BaseWorld.cpp:94-106), which the
         // drivers call before each step (mainIsoElpsPack_WCCM.cpp:129)
-        const Pack& K0 = packs[P.pack[p]];
-        if (p == K0.p_begin) P.maxdisp[P.pack[p]] = 0ULL;
+        const Pack& K0 = packs[ONE ? 0 : P.pack[p]];
+        if (p == K0.p_begin) P.maxdisp[ONE ? 0 : P.pack[p]] = 0ULL;
         if (K0.ctl_rscale != 1.0 && !(P.flags[p] & SKIP_MACRO_SUM) && is_rattler(P, p)) {
             const double sc = K0.ctl_rscale;
             P.vx[p] *= sc; P.vy[p] *= sc; P.w[p] *= sc;
@@ -397,7 +397,7 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
         F.axy[p] = (0.0 - kxy * e2) / bsq;
         F.ayy[p] = (1.0 - kyy * e2) / bsq;
     }
-    const Pack K = packs[P.pack[p]];
+    const Pack K = packs[ONE ? 0 : P.pack[p]];
     double x = P.x[p], y = P.y[p];
     const double x0 = x, y0 = y;
     if (K.world == WORLD_LE) {
@@ -522,6 +522,7 @@ constexpr int kExtraBit = 1 << 30;
 // partner's incoming index.
 constexpr int kMaxList = 48;       // listed pairs j>i per particle
 
+template <bool ONE>
 __global__ void __launch_bounds__(kBlock)
 k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
            const int* __restrict__ rebuild, uint32_t* err, Counters* cnt) {
@@ -536,10 +537,11 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     int nl = 0;
     unsigned long long ncand_total = 0;
     if (active) {
-        const Pack K = packs[P.pack[i]];
+        const int pk = ONE ? 0 : P.pack[i];
+        const Pack K = packs[pk];
         P.dsx[i] = 0.0;
         P.dsy[i] = 0.0;
-        if (i == K.p_begin) { packs[P.pack[i]].shift_acc = 0.0; packs[P.pack[i]].sigma = 1.0; }
+        if (i == K.p_begin) { packs[pk].shift_acc = 0.0; packs[pk].sigma = 1.0; }
         const bool skip_i = K.world == WORLD_PERIODIC && (P.flags[i] & SKIP_CONTACT_DETECTION);
         if (!skip_i) {
             const double xi = P.x[i], yi = P.y[i], ri = P.r[i] + K.skin;
@@ -1348,11 +1350,10 @@ k_log(Parts P, Frame F, const Pack* __restrict__ packs, Table new_t, WorkList W,
 // the incoming entries and the row.
 // Mixed mode: the sums run in FP32.
 template <int PREC>
-__global__ void __launch_bounds__(kBlock)
-k_gather(Parts P, Table t, uint32_t* err) {
+__device__ __forceinline__ void gather_forces(const Parts& P, const Table& t, int p,
+                                              double& fx_out, double& fy_out, double& tq_out,
+                                              int& nc_out) {
     typedef typename std::conditional<PREC == kMixed, float, double>::type acc_t;
-    const int p = xcd_block() * kBlock + threadIdx.x;
-    if (p >= P.n || *err) return;
     const size_t n = (size_t)P.n;
     const int nin = min(P.in_cnt[p], kInCap);
     acc_t fx = 0, fy = 0, tq = 0;
@@ -1376,59 +1377,86 @@ k_gather(Parts P, Table t, uint32_t* err) {
         tq += (acc_t)r.t1;
         ++nc;
     }
-    P.fx[p] = fx; P.fy[p] = fy; P.tq[p] = tq; P.ncont[p] = nc;
-    // setAllParticlesAsRattler, then setAsNonRattler for both partners of
-    // every contact (src/BaseWorld.cpp:25, 44-45)
-    P.rat[p] = nc == 0 ? 1 : 0;
+    fx_out = fx; fy_out = fy; tq_out = tq; nc_out = nc;
 }
 
-// takeTimeIntegral (src/BaseWorld.cpp:59-77, src/BaseParticle.cpp:107-118,
-// clumps src/GluedParticle.cpp:91-125) after removeBackgroudVelocity
-// (src/LeesEdwardsBoundaryWorld.cpp:520-530); then the per-packing LE shift
-// update (src/LeesEdwardsBoundaryWorld.cpp:285-296) and clearing of the
-// per-step bins and the next table's counter.
+// collectForceAndTorque fused with takeTimeIntegral (src/BaseWorld.cpp:59-77,
+// src/BaseParticle.cpp:107-118, clumps src/GluedParticle.cpp:91-125) after
+// removeBackgroudVelocity (src/LeesEdwardsBoundaryWorld.cpp:520-530); then
+// the per-packing LE shift update (src/LeesEdwardsBoundaryWorld.cpp:285-296)
+// and clearing of the per-step bins and the next table's counter.  Each
+// thread sums its particle's contact forces and integrates it; a clump's
+// thread also sums its sub-particles' forces (they are not integrated on
+// their own).  The per-particle force, torque, contact count and rattler
+// flag are stored for the readers on the steps that write contact details
+// (record_flag), and the sums only read the contact list, never particle
+// state, so integrating while other threads still sum is safe.
+template <int PREC, bool ONE>
 __global__ void __launch_bounds__(kBlock)
-k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
-            int* rebuild_next, uint32_t* err) {
+k_integrate(Parts P, Table t, Pack* packs, Bins B, int* next_counter, const int* rebuild,
+            int* rebuild_next, const int* __restrict__ record_flag, uint32_t* err) {
     const int p = xcd_block() * kBlock + threadIdx.x;
     if (*rebuild)
         for (int b = p; b < B.nbins; b += gridDim.x * kBlock) B.count[b] = 0;
     if (p < kSeg) next_counter[p * kCtrStride] = 0;
     if (p >= P.n || *err) return;
-    const int pk = P.pack[p];
+    // the particle's state, loaded before the sums so that the loads overlap
+    const double x0 = P.x[p], y0 = P.y[p], vx0 = P.vx[p], vy0 = P.vy[p];
+    const double w0 = P.w[p], th0 = P.th[p], im = P.im[p], iin = P.iin[p];
+    const double dsx0 = P.dsx[p], dsy0 = P.dsy[p];
+    const int8_t fl = P.flags[p], kd = P.kind[p];
+    const int pk = ONE ? 0 : P.pack[p];
+    double fxp, fyp, tqp;
+    int ncp;
+    gather_forces<PREC>(P, t, p, fxp, fyp, tqp, ncp);
+    if (*record_flag) {
+        P.fx[p] = fxp; P.fy[p] = fyp; P.tq[p] = tqp; P.ncont[p] = ncp;
+        // setAllParticlesAsRattler, then setAsNonRattler for both partners of
+        // every contact (src/BaseWorld.cpp:25, 44-45)
+        P.rat[p] = ncp == 0 ? 1 : 0;
+    }
     const Pack K = packs[pk];
     const double g = K.gdamp, dt = K.dt;
+    double vxr = vx0;
     if (K.world == WORLD_LE) {
-        const double yb = P.y[p];
-        const double vxb = P.vx[p] + K.rate * yb;
-        P.vx[p] = vxb - K.rate * yb;
+        const double vxb = vx0 + K.rate * y0;
+        vxr = vxb - K.rate * y0;
     }
-    if (!(P.flags[p] & SKIP_TIME_INTEGRAL)) {
-        if (P.kind[p] != KIND_CLUMP) {
-            const double im = P.im[p];
-            double vx = P.vx[p], vy = P.vy[p];
-            vx = (vx * (1.0 - g * dt) + (P.fx[p] * im) * dt) / (1.0 + g * dt);
-            vy = (vy * (1.0 - g * dt) + (P.fy[p] * im) * dt) / (1.0 + g * dt);
-            P.vx[p] = vx; P.vy[p] = vy;
-            P.x[p] += vx * dt;
-            P.y[p] += vy * dt;
-            double w = P.w[p];
-            w = (w * (1.0 - g * dt) + P.tq[p] * P.iin[p] * dt) / (1.0 + g * dt);
-            P.w[p] = w;
-            P.th[p] += w * dt;
-            P.dsx[p] += vx * dt;
-            P.dsy[p] += vy * dt;
-            // the reference carries NaN on (0/0 normals of coincident
-            // centres); the engine stops and reports it
-            if (!isfinite(P.x[p]) || !isfinite(P.y[p])) raise_err(err, ERR_NONFINITE);
-        } else {
+    double ddx, ddy;                  // displacement since the list rebuild
+    if (!(fl & SKIP_TIME_INTEGRAL) && kd != KIND_CLUMP) {
+        const double vx = (vxr * (1.0 - g * dt) + (fxp * im) * dt) / (1.0 + g * dt);
+        const double vy = (vy0 * (1.0 - g * dt) + (fyp * im) * dt) / (1.0 + g * dt);
+        P.vx[p] = vx; P.vy[p] = vy;
+        const double x1 = x0 + vx * dt, y1 = y0 + vy * dt;
+        P.x[p] = x1;
+        P.y[p] = y1;
+        const double w = (w0 * (1.0 - g * dt) + tqp * iin * dt) / (1.0 + g * dt);
+        P.w[p] = w;
+        P.th[p] = th0 + w * dt;
+        ddx = dsx0 + vx * dt;
+        ddy = dsy0 + vy * dt;
+        P.dsx[p] = ddx;
+        P.dsy[p] = ddy;
+        // the reference carries NaN on (0/0 normals of coincident
+        // centres); the engine stops and reports it
+        if (!isfinite(x1) || !isfinite(y1)) raise_err(err, ERR_NONFINITE);
+    } else {
+        if (K.world == WORLD_LE) P.vx[p] = vxr;
+        ddx = dsx0;
+        ddy = dsy0;
+    }
+    if (!(fl & SKIP_TIME_INTEGRAL) && kd == KIND_CLUMP) {
+        {
             double tfx = 0.0, tfy = 0.0, tt = 0.0;
             int s = p + 1;
             const int ep = P.eid[p];
             for (; s < P.n && P.up[s] == ep; ++s) {
-                tfx += P.fx[s]; tfy += P.fy[s];
-                tt += P.tq[s];
-                tt += P.cbx[s] * P.fy[s] - P.cby[s] * P.fx[s];
+                double sfx, sfy, stq;
+                int snc;
+                gather_forces<PREC>(P, t, s, sfx, sfy, stq, snc);
+                tfx += sfx; tfy += sfy;
+                tt += stq;
+                tt += P.cbx[s] * sfy - P.cby[s] * sfx;
             }
             // subs belong to this packing; their LE round trip is done here so
             // that the clump's writes below are the last ones
@@ -1438,7 +1466,6 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
                     P.vx[k] = vxb - K.rate * P.y[k];
                 }
             }
-            const double im = P.im[p];
             double vx = P.vx[p], vy = P.vy[p];
             vx = (vx * (1.0 - g * dt) + tfx * im * dt) / (1.0 + g * dt);
             vy = (vy * (1.0 - g * dt) + tfy * im * dt) / (1.0 + g * dt);
@@ -1474,7 +1501,6 @@ k_integrate(Parts P, Pack* packs, Bins B, int* next_counter, const int* rebuild,
     // pair-list validity: a pair can close the skin only if the two
     // particles together travelled it, plus the LE shift change
     const double dshift = K.world == WORLD_LE ? fabs(K.rate * K.ylen * K.dt) : 0.0;
-    const double ddx = P.dsx[p], ddy = P.dsy[p];
     const double disp2 = 2.0 * sqrt(ddx * ddx + ddy * ddy);
     if (K.ctl_mode != 0) {
         // the boundary update of this step still moves every particle:
@@ -2516,9 +2542,9 @@ int launch_step_t(xdem_engine* E) {
     const int par = (int)(E->steps & 1);
     int* rb = E->rebuild_d + par;
     int* rb_next = E->rebuild_d + (par ^ 1);
-    hipLaunchKernelGGL(k_bin<PREC>, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->F,
+    hipLaunchKernelGGL((k_bin<PREC, ONE>), dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->F,
                        E->packs_d, E->B, old_t, new_t, E->W, rb, rb_next, E->err_d);
-    hipLaunchKernelGGL(k_vl_build, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
+    hipLaunchKernelGGL(k_vl_build<ONE>, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
                        E->B, old_t, new_t, rb, E->err_d, E->cnt_d);
     hipLaunchKernelGGL(k_in_sort, dim3(gn), dim3(kBlock), 0, E->stream, E->P, rb, E->err_d);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2543,11 +2569,10 @@ int launch_step_t(xdem_engine* E) {
     if (E->logging)
         hipLaunchKernelGGL(k_log, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P, E->F,
                            E->packs_d, new_t, E->W, E->log, E->err_d);
-    hipLaunchKernelGGL(k_gather<PREC>, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t,
-                       E->err_d);
     const int gi = std::max(gn, grid_of(E->B.nbins) / 4 + 1);
-    hipLaunchKernelGGL(k_integrate, dim3(gi), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
-                       E->B, old_t.counter, rb, rb_next, E->err_d);
+    hipLaunchKernelGGL((k_integrate<PREC, ONE>), dim3(gi), dim3(kBlock), 0, E->stream, E->P,
+                       new_t, E->packs_d, E->B, old_t.counter, rb, rb_next, E->record_d,
+                       E->err_d);
     if (E->control_active) {
         // updateTotalStress, boundary update, rattler reset, position scaling
         hipLaunchKernelGGL(k_stress, dim3(gn), dim3(kBlock), 0, E->stream, E->P, new_t,

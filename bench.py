@@ -262,7 +262,7 @@ def main():
         # on this packing; no result depends on them
         from xdem_amd.autotune import autotune
         best, trials = autotune([s], skin_factors=(0.05, 0.1, 0.15, 0.2, 0.3),
-                                blocks_per_cu=(4, 8, 16), steps=60, warmup=10,
+                                blocks_per_cu=(4, 8, 16), steps=200, warmup=10,
                                 log=lambda m: print(m, file=sys.stderr, flush=True))
         s.cfg.skin = best["skin_factor"] * float(np.max(np.abs(s.r)))
         s.cfg.pair_blocks = best["pair_blocks"]

@@ -492,6 +492,16 @@ __device__ __forceinline__ int cell_list(const Parts& P, const Pack& K, int i, i
     return -1;
 }
 
+// cell_list from P1's cell and P2's two cells.
+__device__ __forceinline__ int cell_list_of(const Pack& K, int ri, int2 cj) {
+    if (in_window(K.nx, ri, cj.x)) return 0;
+    if (cj.y >= 0 && in_window(K.nx, ri, cj.y)) {
+        if (K.world == WORLD_PERIODIC) return 0;
+        return (cj.y >> 16) == 0 ? 2 : 1;
+    }
+    return -1;
+}
+
 // The ID, clump and flag filters of the candidate rule
 // (src/PeriodicBoundaryWorld.cpp:71-72; SKIP_CONTACT_DETECTION in the
 // periodic world only).  They do not change during a run.
@@ -521,6 +531,7 @@ constexpr int kExtraBit = 1 << 30;
 // ordered contact maps are summed, and each pair is entered in its
 // partner's incoming index.
 constexpr int kMaxList = 48;       // listed pairs j>i per particle
+constexpr int kVlBatch = 4;        // candidates loaded together
 
 template <bool ONE>
 __global__ void __launch_bounds__(kBlock)
@@ -546,6 +557,7 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
         if (!skip_i) {
             const double xi = P.x[i], yi = P.y[i], ri = P.r[i] + K.skin;
             const int upi = P.up[i], ki = P.kind[i], eidi = P.eid[i];
+            const int ci = P.cell[i].x;
             const int rci = P.vrc_of[i];
             const int row_i = rci >> 16, col_i = rci & 0xffff;
             int ncand[3] = {0, 0, 0};
@@ -560,25 +572,44 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
                     const int b = K.bin_base + row * K.vnx + jx;
                     const int nb = min(B.count[b], B.cap);
                     const int* lst = B.list + (size_t)b * B.cap;
-                    for (int t = 0; t < nb; ++t) {
-                        const int j = lst[t];
-                        if (eidi >= P.eid[j]) continue;
-                        if (K.world == WORLD_PERIODIC && (P.flags[j] & SKIP_CONTACT_DETECTION)) continue;
-                        if (upi == P.up[j]) continue;
-                        // the reference's own candidate lists (its exit checks)
-                        const int cl = candidate_list(P, K, i, j);
-                        if (cl >= 0) {
-                            ncand[cl]++;
-                            if (ki != KIND_CLUMP && P.kind[j] != ki) raise_err(err, ERR_UNSUPPORTED);
+                    // candidates in groups of kVlBatch: every load of a group is
+                    // issued before any of its results is used
+                    for (int t0 = 0; t0 < nb; t0 += kVlBatch) {
+                        int jj[kVlBatch], ej[kVlBatch], uj[kVlBatch];
+                        int8_t fj[kVlBatch], kj[kVlBatch];
+                        int2 cj[kVlBatch];
+                        double xj[kVlBatch], yj[kVlBatch], rj[kVlBatch];
+#pragma unroll
+                        for (int u = 0; u < kVlBatch; ++u)
+                            jj[u] = t0 + u < nb ? lst[t0 + u] : i;
+#pragma unroll
+                        for (int u = 0; u < kVlBatch; ++u) {
+                            const int j = jj[u];
+                            ej[u] = P.eid[j]; uj[u] = P.up[j]; fj[u] = P.flags[j];
+                            kj[u] = P.kind[j]; cj[u] = P.cell[j];
+                            xj[u] = P.x[j]; yj[u] = P.y[j]; rj[u] = P.r[j];
                         }
-                        if (ki == KIND_CLUMP || P.kind[j] != ki) continue;
-                        const double xj = P.x[j], yj = P.y[j];
-                        double ox, oy, ovx;
-                        image_offset(K, img, xi, yi, xj, yj, ox, oy, ovx);
-                        const double qx = xi - (xj + ox), qy = yi - (yj + oy);
-                        if ((ri + P.r[j]) > sqrt(qx * qx + qy * qy)) {
-                            if (nl < kMaxList) s_list[nl][tid] = j;
-                            ++nl;
+#pragma unroll
+                        for (int u = 0; u < kVlBatch; ++u) {
+                            if (t0 + u >= nb) break;
+                            const int j = jj[u];
+                            if (eidi >= ej[u]) continue;
+                            if (K.world == WORLD_PERIODIC && (fj[u] & SKIP_CONTACT_DETECTION)) continue;
+                            if (upi == uj[u]) continue;
+                            // the reference's own candidate lists (its exit checks)
+                            const int cl = cell_list_of(K, ci, cj[u]);
+                            if (cl >= 0) {
+                                ncand[cl]++;
+                                if (ki != KIND_CLUMP && kj[u] != ki) raise_err(err, ERR_UNSUPPORTED);
+                            }
+                            if (ki == KIND_CLUMP || kj[u] != ki) continue;
+                            double ox, oy, ovx;
+                            image_offset(K, img, xi, yi, xj[u], yj[u], ox, oy, ovx);
+                            const double qx = xi - (xj[u] + ox), qy = yi - (yj[u] + oy);
+                            if ((ri + rj[u]) > sqrt(qx * qx + qy * qy)) {
+                                if (nl < kMaxList) s_list[nl][tid] = j;
+                                ++nl;
+                            }
                         }
                     }
                 }

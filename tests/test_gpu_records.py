@@ -55,3 +55,4 @@ def test_single_step_calls_match_one_call(name, cell_factor):
             sc = max(float(np.max(np.abs(co[k]))), 1e-300)
             assert float(np.max(np.abs(cb[k] - co[k]))) <= 1e-6 * sc, k
     a.close(); b.close()
+

@@ -764,6 +764,97 @@ __device__ __forceinline__ bool pw_separated(T dx, T dy, double a1, double b1, d
     return q > (T)2 * ((T)1 + margin) * det;
 }
 
+// One listed pair of the coarse pass: its history in the previous contact
+// list, the candidate rule and image of this step, the bounding-circle test
+// and the separation pre-test.  Returns true with the work item when the
+// pair goes on to the contact computation; otherwise erases its history or
+// carries it forward (link[e]).  pr_i / pr_j are shared by both tables
+// (rewritten only at rebuilds).
+template <int PREC, bool ONE>
+__device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
+                                            const Pack* __restrict__ packs, const Table& old_t,
+                                            const Table& new_t, bool rb, int e, uint32_t* err,
+                                            unsigned long long& n_coarse, int4& item,
+                                            float (&feat)[kMlpIn]) {
+    bool push = false;
+    const int i = new_t.pr_i[e];
+    const int code = new_t.pr_j[e];
+    const int j = code & ~kExtraBit;
+    if (i >= 0 && i < P.n && j >= 0 && j < P.n) {
+        const Pack K = packs[ONE ? 0 : P.pack[i]];
+        // history of (i, j) as an index into the previous contact
+        // list: the same slot when the list was not rebuilt
+        int found = -1;
+        if (!rb) {
+            found = old_t.link[e];
+        } else {
+            const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
+            for (int t = os; t < os + oc; ++t) {
+                const int c = old_t.link[t];
+                if (c >= 0 && old_t.cj[c] == j) { found = c; break; }
+            }
+        }
+        const int ki = P.kind[i];
+        int lst;
+        bool shapes = true;
+        if (code & kExtraBit) {
+            lst = candidate_list(P, K, i, j);
+            shapes = ki != KIND_CLUMP && P.kind[j] == ki;
+        } else {
+            lst = cell_list(P, K, i, j);
+        }
+        const double rsum = P.r[i] + P.r[j];
+        if (lst >= 0 && shapes) {
+            const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
+            double ox, oy, ovx;
+            image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
+            if (PREC == kMixed) {
+                const float dxf = (float)((xj + ox) - xi), dyf = (float)((yj + oy) - yi);
+                const float rs = (float)rsum * kCullMargin;
+                push = rs >= sqrtf(dxf * dxf + dyf * dyf);
+                n_coarse += (ki == KIND_CIRCLE || push) ? 1 : 0;
+                if (push && ki == KIND_ELLIPSE && !K.strict)
+                    push = !pw_separated<float>(dxf, dyf, P.r[i], F.b[i], F.kx[i], F.ky[i],
+                                                P.r[j], F.b[j], F.kx[j], F.ky[j],
+                                                kPwMarginF32);
+            } else if (ki == KIND_CIRCLE) {
+                const double bvx = (xj + ox) - xi, bvy = (yj + oy) - yi;
+                push = !(rsum - sqrt(bvx * bvx + bvy * bvy) < 0.0);
+                ++n_coarse;
+            } else {
+                const double qx = xi - (xj + ox), qy = yi - (yj + oy);
+                push = rsum > sqrt(qx * qx + qy * qy);
+                n_coarse += push ? 1 : 0;
+                if (push && !K.strict)
+                    push = !pw_separated<double>(-qx, -qy, P.r[i], F.b[i], F.kx[i], F.ky[i],
+                                                 P.r[j], F.b[j], F.kx[j], F.ky[j],
+                                                 kPwMargin);
+                if (PREC == kNN && push)
+                    root_features((float)((xj + ox) - xi), (float)((yj + oy) - yi),
+                                  (float)F.kx[i], (float)F.ky[i], (float)F.kx[j],
+                                  (float)F.ky[j], (float)P.r[i], (float)F.b[i],
+                                  (float)P.r[j], (float)F.b[j], feat);
+            }
+        }
+        if (push) {
+            // a contact that the next step could carry forward unevaluated
+            // (contact_item)
+            const bool risky = rsum >= K.risk_r;
+            item = make_int4((int)((unsigned)e | ((unsigned)lst << kLstShift)),
+                             item_y(found, risky), i, j);
+        } else if (found >= 0 && lst < 0) {     // not a candidate: carried forward
+            const int c = alloc_one(new_t.ccounter, new_t.cseg_cap, err);
+            if (c >= 0) copy_contact(old_t, found, new_t, c);
+            new_t.link[e] = c;
+        } else {
+            new_t.link[e] = -1;
+        }
+    } else {
+        new_t.link[e] = -1;
+    }
+    return push;
+}
+
 // ONE: the engine holds a single packing, so every lane reads the same
 // packing record (scalar loads instead of one vector load per field).
 template <int PREC, bool ONE>
@@ -775,9 +866,6 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
     __shared__ int s_base;
     __shared__ int s_go;
     const bool rb = *rebuild != 0;
-    // pr_i / pr_j are shared by both tables (rewritten only at rebuilds)
-    const int* __restrict__ src_i = new_t.pr_i;
-    const int* __restrict__ src_j = new_t.pr_j;
     if (threadIdx.x == 0) {
         s_go = *err == 0;
         segment_prefix(s_pre, rb ? new_t.counter : old_t.counter, new_t.seg_cap);
@@ -798,81 +886,8 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
         if (w < wend) {
             const int lo = segment_of(s_pre, w);
             const int e = lo * new_t.seg_cap + (w - s_pre[lo]);
-            const int i = src_i[e];
-            const int code = src_j[e];
-            const int j = code & ~kExtraBit;
-            if (i >= 0 && i < P.n && j >= 0 && j < P.n) {
-                const Pack K = packs[ONE ? 0 : P.pack[i]];
-                // history of (i, j) as an index into the previous contact
-                // list: the same slot when the list was not rebuilt
-                int found = -1;
-                if (!rb) {
-                    found = old_t.link[e];
-                } else {
-                    const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
-                    for (int t = os; t < os + oc; ++t) {
-                        const int c = old_t.link[t];
-                        if (c >= 0 && old_t.cj[c] == j) { found = c; break; }
-                    }
-                }
-                const int ki = P.kind[i];
-                int lst;
-                bool shapes = true;
-                if (code & kExtraBit) {
-                    lst = candidate_list(P, K, i, j);
-                    shapes = ki != KIND_CLUMP && P.kind[j] == ki;
-                } else {
-                    lst = cell_list(P, K, i, j);
-                }
-                const double rsum = P.r[i] + P.r[j];
-                if (lst >= 0 && shapes) {
-                    const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
-                    double ox, oy, ovx;
-                    image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
-                    if (PREC == kMixed) {
-                        const float dxf = (float)((xj + ox) - xi), dyf = (float)((yj + oy) - yi);
-                        const float rs = (float)rsum * kCullMargin;
-                        push = rs >= sqrtf(dxf * dxf + dyf * dyf);
-                        n_coarse += (ki == KIND_CIRCLE || push) ? 1 : 0;
-                        if (push && ki == KIND_ELLIPSE && !K.strict)
-                            push = !pw_separated<float>(dxf, dyf, P.r[i], F.b[i], F.kx[i], F.ky[i],
-                                                        P.r[j], F.b[j], F.kx[j], F.ky[j],
-                                                        kPwMarginF32);
-                    } else if (ki == KIND_CIRCLE) {
-                        const double bvx = (xj + ox) - xi, bvy = (yj + oy) - yi;
-                        push = !(rsum - sqrt(bvx * bvx + bvy * bvy) < 0.0);
-                        ++n_coarse;
-                    } else {
-                        const double qx = xi - (xj + ox), qy = yi - (yj + oy);
-                        push = rsum > sqrt(qx * qx + qy * qy);
-                        n_coarse += push ? 1 : 0;
-                        if (push && !K.strict)
-                            push = !pw_separated<double>(-qx, -qy, P.r[i], F.b[i], F.kx[i], F.ky[i],
-                                                         P.r[j], F.b[j], F.kx[j], F.ky[j],
-                                                         kPwMargin);
-                        if (PREC == kNN && push)
-                            root_features((float)((xj + ox) - xi), (float)((yj + oy) - yi),
-                                          (float)F.kx[i], (float)F.ky[i], (float)F.kx[j],
-                                          (float)F.ky[j], (float)P.r[i], (float)F.b[i],
-                                          (float)P.r[j], (float)F.b[j], feat);
-                    }
-                }
-                if (push) {
-                    // a contact that the next step could carry forward unevaluated
-                    // (contact_item)
-                    const bool risky = rsum >= K.risk_r;
-                    item = make_int4((int)((unsigned)e | ((unsigned)lst << kLstShift)),
-                                     item_y(found, risky), i, j);
-                } else if (found >= 0 && lst < 0) {     // not a candidate: carried forward
-                    const int c = alloc_one(new_t.ccounter, new_t.cseg_cap, err);
-                    if (c >= 0) copy_contact(old_t, found, new_t, c);
-                    new_t.link[e] = c;
-                } else {
-                    new_t.link[e] = -1;
-                }
-            } else {
-                new_t.link[e] = -1;
-            }
+            push = coarse_slot<PREC, ONE>(P, F, packs, old_t, new_t, rb, e, err, n_coarse, item,
+                                          feat);
         }
         // block compaction: one counter update per block and round
         const unsigned long long m = __ballot(push ? 1 : 0);
@@ -2588,14 +2603,15 @@ int launch_step_t(xdem_engine* E) {
         e0 = E->ev_pool[E->ev_used++];
         e1 = E->ev_pool[E->ev_used++];
     }
-    hipLaunchKernelGGL((k_coarse<PREC, ONE>), dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
-                       E->F, E->packs_d, old_t, new_t, E->W, rb, E->err_d, E->cnt_d);
+    hipLaunchKernelGGL((k_coarse<PREC, ONE>), dim3(E->pair_grid), dim3(kBlock), 0, E->stream,
+                       E->P, E->F, E->packs_d, old_t, new_t, E->W, rb, E->err_d, E->cnt_d);
     if (PREC == kNN)
         hipLaunchKernelGGL(k_guess, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->W,
                            E->mlp_pk, E->err_d);
     if (E->timing) HIPCHK(hipEventRecord(e0, E->stream));
-    hipLaunchKernelGGL((k_contact<PREC, ONE>), dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P,
-                       E->F, E->packs_d, old_t, new_t, E->W, E->record_d, E->err_d, E->cnt_d);
+    hipLaunchKernelGGL((k_contact<PREC, ONE>), dim3(E->pair_grid), dim3(kBlock), 0, E->stream,
+                       E->P, E->F, E->packs_d, old_t, new_t, E->W, E->record_d, E->err_d,
+                       E->cnt_d);
     if (E->timing) HIPCHK(hipEventRecord(e1, E->stream));
     if (E->logging)
         hipLaunchKernelGGL(k_log, dim3(E->pair_grid), dim3(kBlock), 0, E->stream, E->P, E->F,

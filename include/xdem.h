@@ -106,7 +106,9 @@ const char* xdem_last_error(const xdem_engine* e);
  * entries (NULL allowed for one packing).  mass and inertia are the values
  * the reference's calculateParticleProperties produces; clump_bx/by (may be
  * NULL) are a clump sub-particle's branch vector from its clump centre
- * (src/GluedParticle.cpp:71-72), unused for other particles. */
+ * (src/GluedParticle.cpp:71-72), unused for other particles.  Non-finite
+ * input gives XDEM_E_NONFINITE (the reference would run on with it), a
+ * packing narrower than 3 reference cells XDEM_E_GRID. */
 int xdem_set_particles(xdem_engine* e, int n, const int* packing_start,
                        const int* kind, const int* up_level, const int* flags,
                        const double* outer_r, const double* aspect,

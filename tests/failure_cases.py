@@ -1,0 +1,66 @@
+"""States that drive the engine and the oracle into the reference's exit
+paths and the engine's own refusals (shared by the CPU and GPU failure
+tests)."""
+import numpy as np
+
+from tests import helpers
+from xdem_amd.state import (KIND_CIRCLE, KIND_ELLIPSE, WORLD_PERIODIC, State, WorldConfig,
+                            particle_properties)
+
+
+def _state(kind, r, ar, x, y, th, L, **cfg):
+    n = len(r)
+    kind = np.asarray(kind, np.int32)
+    r, ar = np.asarray(r, np.float64), np.asarray(ar, np.float64)
+    m, inertia, _ = particle_properties(kind, r, ar)
+    c = WorldConfig(world_kind=WORLD_PERIODIC, xmin=-L / 2, xmax=L / 2, ymin=-L / 2,
+                    ymax=L / 2, dt=1e-4, kn=1e4, kt=5e3, restitution=0.5, mu=0.5, **cfg)
+    z = np.zeros(n)
+    return State(c, kind, np.arange(n, dtype=np.int32), np.zeros(n, np.int32), r, ar, m,
+                 inertia, np.asarray(x, np.float64), np.asarray(y, np.float64),
+                 np.asarray(th, np.float64), z.copy(), z.copy(), z.copy(), z.copy(), z.copy())
+
+
+def candidate_overflow():
+    """One circle of radius 5 sets the cell size to 10.05; 260 circles of
+    radius 0.02 in a 4 x 4 square then give each of them more than 199
+    candidates (reference: exit(-2), src/PeriodicBoundaryWorld.cpp:75-78)."""
+    rng = np.random.default_rng(11)
+    n = 261
+    x = np.concatenate([[-12.0], rng.uniform(2.0, 6.0, n - 1)])
+    y = np.concatenate([[0.0], rng.uniform(-2.0, 2.0, n - 1)])
+    r = np.concatenate([[5.0], np.full(n - 1, 0.02)])
+    return _state(np.full(n, KIND_CIRCLE), r, np.ones(n), x, y, np.zeros(n), 40.0)
+
+
+def unlike_shapes():
+    """A circle among ellipses (reference: exit(-1), src/EllipticParticle.cpp:30)."""
+    s = helpers.state("pb_ellipse")
+    s.kind[:] = KIND_ELLIPSE
+    s.kind[0] = KIND_CIRCLE
+    return s
+
+
+# the pose of tests/test_oracle_math.py::test_newton_cycle_rescue: the
+# reference's Newton loop cycles forever on it
+CYCLE_POSE = [0.4321304017550254, 1.7999892628071283, -1.7702608555070118, 6.258542045844655,
+              0.18454795164527532, 0.5624894201105685, 1.2857174827581246,
+              -2.1527440943683214, 6.634985207575501, 2.5352734119304268]
+
+
+def newton_cycle(strict):
+    p = CYCLE_POSE
+    return _state([KIND_ELLIPSE] * 2, [p[0], p[5]], [p[1], p[6]], [p[2], p[7]],
+                  [p[3], p[8]], [p[4], p[9]], 20.0, strict_newton=int(strict))
+
+
+def small_grid():
+    """A box less than three cells wide (the reference visits a cell twice)."""
+    return _state([KIND_CIRCLE] * 2, [0.5, 0.5], [1.0, 1.0], [-0.6, 0.6], [0.0, 0.0],
+                  [0.0, 0.0], 2.5)
+
+
+def nonfinite_input():
+    s = helpers.state("pb_circle")
+    s.x[3] = np.nan
+    return s

@@ -1,0 +1,37 @@
+"""The reference's exit paths on the CPU oracle: each condition at which
+the reference prints and calls exit() is forced from the host and comes
+back as the C-ABI status code (tests/failure_cases.py builds the states)."""
+import pytest
+
+from oracle import pyoracle as po
+from tests import failure_cases as fc
+
+
+def _code(s):
+    w = po.OracleWorld(s)
+    with pytest.raises(po.OracleError) as ei:
+        w.step(1)
+    return ei.value.code
+
+
+def test_candidate_overflow():
+    assert _code(fc.candidate_overflow()) == -2        # XDEM_E_CAPACITY
+
+
+def test_unlike_shapes():
+    assert _code(fc.unlike_shapes()) == -3             # XDEM_E_UNSUPPORTED_PAIR
+
+
+def test_newton_cycle_strict_and_rescued():
+    assert _code(fc.newton_cycle(True)) == -6          # XDEM_E_NEWTON_CAP
+    w = po.OracleWorld(fc.newton_cycle(False))
+    w.step(1)
+    assert w.counters()["rescues"] >= 1
+    assert len(w.contacts()["i"]) == 1
+
+
+def test_small_grid():
+    with pytest.raises(po.OracleError) as ei:
+        w = po.OracleWorld(fc.small_grid())
+        w.step(1)
+    assert ei.value.code == -8                          # XDEM_E_GRID

@@ -1,0 +1,62 @@
+"""The reference's exit paths through the C-ABI on the GPU: each condition
+is forced from the host (tests/failure_cases.py) and every kernel still
+exits normally; the step reports the status code, as the oracle does
+(tests/test_failures.py).  Non-finite input and a grid below 3 x 3 are
+refused at xdem_set_particles."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import failure_cases as fc
+
+pytestmark = pytest.mark.gpu
+
+
+def _step_code(s, steps=1):
+    from xdem_amd.engine import Engine, XdemError
+    eng = Engine(s)
+    with pytest.raises(XdemError) as ei:
+        eng.step(steps)
+    code = ei.value.code
+    # the error stays reported
+    with pytest.raises(XdemError) as ei2:
+        eng.particles()
+    assert ei2.value.code == code
+    eng.close()
+    return code
+
+
+def test_candidate_overflow():
+    print("[gpu] failure: candidate overflow", flush=True)
+    assert _step_code(fc.candidate_overflow()) == -2
+
+
+def test_unlike_shapes():
+    print("[gpu] failure: unlike shapes", flush=True)
+    assert _step_code(fc.unlike_shapes()) == -3
+
+
+def test_newton_cycle_strict_and_rescued():
+    from xdem_amd.engine import Engine
+    print("[gpu] failure: Newton cycle, strict", flush=True)
+    assert _step_code(fc.newton_cycle(True)) == -6
+    print("[gpu] Newton cycle, rescued", flush=True)
+    s = fc.newton_cycle(False)
+    eng, orc = Engine(s), po.OracleWorld(s)
+    eng.step(1)
+    orc.step(1)
+    assert eng.counters()["rescues"] >= 1
+    ce, co = eng.contacts(), orc.contacts()
+    assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"])
+    for k in ("n_x", "n_y", "point_x", "point_y", "fn_x", "fn_y"):
+        assert np.allclose(ce[k], co[k], rtol=1e-12, atol=1e-12), k
+    eng.close()
+
+
+@pytest.mark.parametrize("case,code", [("small_grid", -8), ("nonfinite_input", -5)])
+def test_refused_at_set_particles(case, code):
+    from xdem_amd.engine import Engine, XdemError
+    print(f"[gpu] failure: {case}", flush=True)
+    with pytest.raises(XdemError) as ei:
+        Engine(getattr(fc, case)())
+    assert ei.value.code == code

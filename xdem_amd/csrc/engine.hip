@@ -2276,6 +2276,15 @@ int xdem_set_particles(xdem_engine* E, int n, const int* packing_start, const in
     if (ps[0] != 0 || ps[E->npacks] != n) return fail(E, XDEM_E_INVALID, "bad packing_start");
     for (int k = 0; k < E->npacks; ++k)
         if (ps[k + 1] <= ps[k]) return fail(E, XDEM_E_INVALID, "empty packing");
+    // the reference runs on with NaN or infinite input; the engine refuses it
+    for (int p = 0; p < n; ++p) {
+        const double v[10] = {outer_r[p], aspect[p], mass[p], inertia[p], x[p], y[p],
+                              theta[p], vx[p], vy[p], omega[p]};
+        for (double q : v)
+            if (!std::isfinite(q))
+                return fail(E, XDEM_E_NONFINITE,
+                            "particle " + std::to_string(p) + ": non-finite input");
+    }
     E->n = n;
     E->pack_start = ps;
     // per-packing grid (src/PeriodicBoundaryWorld.cpp:15-22) and parameters

@@ -47,6 +47,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxCand = 200;      // src/BaseWorld.h:11
 constexpr int kInCap = 32;         // contacts per particle held by partners
+constexpr int kGatherBatch = 4;    // force-sum entries loaded together
 constexpr int kImgShift = 30;
 // The pair/contact table is split into kSeg segments with one bump counter
 // each (kCtrStride ints apart, separate cache lines): thousands of
@@ -1402,26 +1403,47 @@ __device__ __forceinline__ void gather_forces(const Parts& P, const Table& t, in
     typedef typename std::conditional<PREC == kMixed, float, double>::type acc_t;
     const size_t n = (size_t)P.n;
     const int nin = min(P.in_cnt[p], kInCap);
+    const int s = t.row_start[p], cnt = t.row_cnt[p];
     acc_t fx = 0, fy = 0, tq = 0;
     int nc = 0;
-    for (int k = 0; k < nin; ++k) {
-        const int c = t.link[P.in_ref[k * n + p].y];
-        if (c < 0) continue;
-        const FRec r = t.fr[c];
-        fx += -(acc_t)r.fx;
-        fy += -(acc_t)r.fy;
-        tq += (acc_t)r.t2;
-        ++nc;
+    // entries in groups of kGatherBatch: the loads of a group (slot, link,
+    // force record) are issued together; the sums keep the entry order
+    for (int k0 = 0; k0 < nin; k0 += kGatherBatch) {
+        int c[kGatherBatch];
+        FRec r[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u)
+            c[u] = k0 + u < nin ? P.in_ref[(k0 + u) * n + p].y : -1;
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) c[u] = c[u] >= 0 ? t.link[c[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u)
+            if (c[u] >= 0) r[u] = t.fr[c[u]];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) {
+            if (c[u] < 0) continue;
+            fx += -(acc_t)r[u].fx;
+            fy += -(acc_t)r[u].fy;
+            tq += (acc_t)r[u].t2;
+            ++nc;
+        }
     }
-    const int s = t.row_start[p], cnt = t.row_cnt[p];
-    for (int e = s; e < s + cnt; ++e) {
-        const int c = t.link[e];
-        if (c < 0) continue;
-        const FRec r = t.fr[c];
-        fx += (acc_t)r.fx;
-        fy += (acc_t)r.fy;
-        tq += (acc_t)r.t1;
-        ++nc;
+    for (int e0 = s; e0 < s + cnt; e0 += kGatherBatch) {
+        int c[kGatherBatch];
+        FRec r[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) c[u] = e0 + u < s + cnt ? t.link[e0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u)
+            if (c[u] >= 0) r[u] = t.fr[c[u]];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) {
+            if (c[u] < 0) continue;
+            fx += (acc_t)r[u].fx;
+            fy += (acc_t)r[u].fy;
+            tq += (acc_t)r[u].t1;
+            ++nc;
+        }
     }
     fx_out = fx; fy_out = fy; tq_out = tq; nc_out = nc;
 }

@@ -783,6 +783,15 @@ __device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
     const int j = code & ~kExtraBit;
     if (i >= 0 && i < P.n && j >= 0 && j < P.n) {
         const Pack K = packs[ONE ? 0 : P.pack[i]];
+        // every load the tests below may need, issued before any is used
+        const int rci = P.cell[i].x;
+        const int2 cj2 = P.cell[j];
+        const int ki = P.kind[i];
+        const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
+        const double ri = P.r[i], rj = P.r[j];
+        const double rsum = ri + rj;
+        const double fbi = F.b[i], fkxi = F.kx[i], fkyi = F.ky[i];
+        const double fbj = F.b[j], fkxj = F.kx[j], fkyj = F.ky[j];
         // history of (i, j) as an index into the previous contact
         // list: the same slot when the list was not rebuilt
         int found = -1;
@@ -795,18 +804,15 @@ __device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
                 if (c >= 0 && old_t.cj[c] == j) { found = c; break; }
             }
         }
-        const int ki = P.kind[i];
         int lst;
         bool shapes = true;
         if (code & kExtraBit) {
             lst = candidate_list(P, K, i, j);
             shapes = ki != KIND_CLUMP && P.kind[j] == ki;
         } else {
-            lst = cell_list(P, K, i, j);
+            lst = cell_list_of(K, rci, cj2);
         }
-        const double rsum = P.r[i] + P.r[j];
         if (lst >= 0 && shapes) {
-            const double xi = P.x[i], yi = P.y[i], xj = P.x[j], yj = P.y[j];
             double ox, oy, ovx;
             image_offset(K, lst, xi, yi, xj, yj, ox, oy, ovx);
             if (PREC == kMixed) {
@@ -815,9 +821,8 @@ __device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
                 push = rs >= sqrtf(dxf * dxf + dyf * dyf);
                 n_coarse += (ki == KIND_CIRCLE || push) ? 1 : 0;
                 if (push && ki == KIND_ELLIPSE && !K.strict)
-                    push = !pw_separated<float>(dxf, dyf, P.r[i], F.b[i], F.kx[i], F.ky[i],
-                                                P.r[j], F.b[j], F.kx[j], F.ky[j],
-                                                kPwMarginF32);
+                    push = !pw_separated<float>(dxf, dyf, ri, fbi, fkxi, fkyi, rj, fbj, fkxj,
+                                                fkyj, kPwMarginF32);
             } else if (ki == KIND_CIRCLE) {
                 const double bvx = (xj + ox) - xi, bvy = (yj + oy) - yi;
                 push = !(rsum - sqrt(bvx * bvx + bvy * bvy) < 0.0);
@@ -827,14 +832,12 @@ __device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
                 push = rsum > sqrt(qx * qx + qy * qy);
                 n_coarse += push ? 1 : 0;
                 if (push && !K.strict)
-                    push = !pw_separated<double>(-qx, -qy, P.r[i], F.b[i], F.kx[i], F.ky[i],
-                                                 P.r[j], F.b[j], F.kx[j], F.ky[j],
-                                                 kPwMargin);
+                    push = !pw_separated<double>(-qx, -qy, ri, fbi, fkxi, fkyi, rj, fbj, fkxj,
+                                                 fkyj, kPwMargin);
                 if (PREC == kNN && push)
                     root_features((float)((xj + ox) - xi), (float)((yj + oy) - yi),
-                                  (float)F.kx[i], (float)F.ky[i], (float)F.kx[j],
-                                  (float)F.ky[j], (float)P.r[i], (float)F.b[i],
-                                  (float)P.r[j], (float)F.b[j], feat);
+                                  (float)fkxi, (float)fkyi, (float)fkxj, (float)fkyj,
+                                  (float)ri, (float)fbi, (float)rj, (float)fbj, feat);
             }
         }
         if (push) {
@@ -1247,10 +1250,20 @@ k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table n
     int wbeg, wend;
     xcd_chunk(s_pre[kSeg], wbeg, wend);
     StepCounts n;
+    // the next item is loaded while the current one is computed
+    int4 nxt = make_int4(0, 0, 0, 0);
+    if (wbeg + (int)threadIdx.x < wend) {
+        const int lo = segment_of(s_pre, wbeg + threadIdx.x);
+        nxt = W.item[lo * W.seg_cap + (wbeg + threadIdx.x - s_pre[lo])];
+    }
     for (int w = wbeg + threadIdx.x; w < wend; w += kBlock) {
         const int lo = segment_of(s_pre, w);
         const int at = lo * W.seg_cap + (w - s_pre[lo]);
-        const int4 it = W.item[at];
+        const int4 it = nxt;
+        if (w + kBlock < wend) {
+            const int lo2 = segment_of(s_pre, w + kBlock);
+            nxt = W.item[lo2 * W.seg_cap + (w + kBlock - s_pre[lo2])];
+        }
         const unsigned ux = (unsigned)it.x;
         const int e = (int)(ux & ((1u << kLstShift) - 1u));
         const int lst = (int)(ux >> kLstShift);

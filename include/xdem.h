@@ -227,6 +227,16 @@ typedef struct xdem_control {
  * steps).  The boundary velocities start at zero, as in the reference. */
 int xdem_set_control(xdem_engine* e, int packing, const xdem_control* c);
 
+/* One boundary update of the strain mode applied now, between steps:
+ * PeriodicControlWorld::updatePeriodicBoundary_strainControl(rate_x, rate_y)
+ * (src/PeriodicControlWorld.cpp:124-135), which scales the box by
+ * 1 + rate * length * dt and every position of the packing with it.  For
+ * protocols that decide from the observables of a step whether to compress
+ * (mainIsoElpsPackTestEntropy.cpp:151-171).  The packing's control settings
+ * are left as they were. */
+int xdem_apply_strain(xdem_engine* e, int packing, double strain_rate_x,
+                      double strain_rate_y);
+
 /* Box of each packing: out[10k..10k+9] = xmin, xmax, ymin, ymax, boundary
  * velocity x, y (_velX, _velY) and the total stress xx, xy, yx, yy the last
  * control step followed (src/PeriodicControlWorld.h:8-11, src/BaseWorld.h). */

@@ -50,6 +50,7 @@ def lib():
         L.oracle_num_contacts.argtypes = [C.c_void_p]
         L.oracle_get_contacts.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _I, _I] + [_D] * 13
         L.oracle_get_branches.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)] + [_D] * 6
+        L.oracle_apply_strain.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.oracle_get_stress.argtypes = [C.c_void_p, _D]
         L.oracle_get_boundary.argtypes = [C.c_void_p, _D]
         L.oracle_get_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_longlong)] * 5
@@ -156,6 +157,10 @@ class OracleWorld:
                                              float(rattler_scale), float(stress),
                                              float(rate_x), float(rate_y),
                                              float(boundary_mass), float(max_strain_rate)))
+
+    def apply_strain(self, rate_x, rate_y):
+        """One strain-control boundary update now (see xdem_apply_strain)."""
+        self._check(lib().oracle_apply_strain(self.h, float(rate_x), float(rate_y)))
 
     def observables(self):
         """countPossibleContacts, countActucalContacts, getKineticEnergy,
@@ -267,9 +272,11 @@ def read_golden(path):
 
 def read_control_golden(path):
     """Parse a ref_harness `control` dump: per step the `run` record plus
-    `box` = (xmin, xmax, ymin, ymax, vel x, vel y, control stress xx, xy, yx, yy)."""
+    `box` = (xmin, xmax, ymin, ymax, vel x, vel y, control stress xx, xy, yx, yy);
+    a `quasistatic` dump (XDEMGQ01) adds `moved` (1.0 when the boundary moved)."""
     buf = open(path, "rb").read()
-    assert buf[:8] == b"XDEMGC01"
+    assert buf[:8] in (b"XDEMGC01", b"XDEMGQ01")
+    extra = 1 if buf[:8] == b"XDEMGQ01" else 0
     tmp = path + ".gd"
     steps = []
     off = 8
@@ -285,8 +292,10 @@ def read_control_golden(path):
             f.write(b"XDEMGD02" + body[pos:end])
         d = read_golden(tmp)[0]
         d["box"] = np.frombuffer(body, np.float64, 10, end).copy()
+        if extra:
+            d["moved"] = float(np.frombuffer(body, np.float64, 1, end + 80)[0])
         steps.append(d)
-        pos = end + 80
+        pos = end + 80 + 8 * extra
     os.remove(tmp)
     return steps
 
@@ -299,6 +308,12 @@ def run_reference_control(state_path, out_path, nsteps, mode, stress=0.0, rate_x
                     repr(float(rattler_scale)), str(int(bool(reset_rattlers))),
                     repr(float(boundary_mass)), repr(float(max_strain_rate))],
                    check=True, stdout=subprocess.DEVNULL, timeout=300)
+    return read_control_golden(out_path)
+
+
+def run_reference_quasistatic(state_path, out_path, nsteps):
+    subprocess.run([HARNESS, "quasistatic", state_path, out_path, str(nsteps)], check=True,
+                   stdout=subprocess.DEVNULL, timeout=300)
     return read_control_golden(out_path)
 
 

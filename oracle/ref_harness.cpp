@@ -32,6 +32,12 @@
 //       read4File_particle src/BaseParticle.cpp:173-186), the boundary and
 //       Lees-Edwards shift of the first, and runs m steps, dumped after each.
 //       Ellipses only (the circle's stream constructor sets no contact model).
+//   ref_harness quasistatic <state.xst> <out.xdg> <nsteps>
+//       the compression loop of mainIsoElpsPackTestEntropy.cpp:140-180 in a
+//       PeriodicControlWorld: the step's parts, scaleVelocityOfRattlers(0.95),
+//       the observables, and updatePeriodicBoundary_strainControl(-0.2, -0.2)
+//       whenever the driver's criterion holds; dumps every step, then the
+//       boundary record and 1.0 when the boundary moved (magic XDEMGQ01).
 //
 // The state file layout is written by xdem_amd/state.py (write_state).
 #include <chrono>
@@ -427,6 +433,58 @@ int mode_checkpoint(int argc, char** argv) {
     return 0;
 }
 
+int mode_quasistatic(int argc, char** argv) {
+    if (argc < 5) return 2;
+    Built B;
+    B.s = read_state(argv[2]);
+    const int nsteps = atoi(argv[4]);
+    build(B, 1, 100, true);
+    HPC& W = *B.pc;
+    W.updateTotalParticleVolume();
+    W.setVirtualBoundaryMass();
+    FILE* f = fopen(argv[3], "wb");
+    if (!f) return 3;
+    fwrite("XDEMGQ01", 1, 8, f);
+    int count = 0, count0 = 0;
+    bool goNextComp = true;
+    const int critCntNum = B.s.n * 2.5;
+    for (int s = 1; s <= nsteps; ++s) {
+        W.modifyParticlePosition();
+        W.findPossibleContacts();
+        W.updateContacts();
+        W.collectForceAndTorque();
+        W.takeTimeIntegral();
+        W.scaleVelocityOfRattlers(0.95);
+        const double elasticEnergy = W.getElasticEnergyPerContact();
+        const double kineticEnergy = W.getKineticEnergyPerNonRattlerParticle();
+        const int actualContactNum = W.countActucalContacts();
+        double moved = 0.0;
+        if ((elasticEnergy < 1e-6 && actualContactNum < critCntNum && goNextComp) ||
+            (elasticEnergy >= 1e-6 && actualContactNum >= critCntNum && kineticEnergy < 5e-2 &&
+             goNextComp)) {
+            if (count == 1) {
+                W.updateTotalStress();
+                count = 0;
+                count0 = 0;
+            }
+            W.updatePeriodicBoundary_strainControl(-2e-1, -2e-1);
+            count++;
+            goNextComp = false;
+            moved = 1.0;
+        } else {
+            count0++;
+            if (count0 > 10) goNextComp = true;
+        }
+        double bx[10];
+        W.box(bx);
+        dump(B, f, s);
+        wr(f, bx, 10);
+        wr(f, &moved, 1);
+    }
+    fclose(f);
+    return 0;
+}
+
 int mode_control(int argc, char** argv) {
     if (argc < 13) return 2;
     Built B;
@@ -481,5 +539,6 @@ int main(int argc, char** argv) {
     if (m == "control") return mode_control(argc, argv);
     if (m == "history") return mode_history(argc, argv);
     if (m == "checkpoint") return mode_checkpoint(argc, argv);
+    if (m == "quasistatic") return mode_quasistatic(argc, argv);
     return 2;
 }

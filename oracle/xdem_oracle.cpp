@@ -997,6 +997,20 @@ int oracle_get_stress(void* h, double* out) {
 // Boundary control, mirrors xdem_set_control: mode 1 strain, 2 stress
 // uniform, 3 stress X then Y, 4 stress uniform with strain limit.
 // mass <= 0: setVirtualBoundaryMass() (src/PeriodicControlWorld.cpp:58-66).
+// One strain-control boundary update now, between steps (mirrors
+// xdem_apply_strain): updatePeriodicBoundary_strainControl,
+// src/PeriodicControlWorld.cpp:124-135.
+int oracle_apply_strain(void* h, double rate_x, double rate_y) {
+    World& W = *(World*)h;
+    if (W.world_kind != WORLD_PERIODIC) return XDEM_E_INVALID;
+    const int mode = W.ctl_mode;
+    const double rx = W.ctl_rx, ry = W.ctl_ry;
+    W.ctl_mode = 1; W.ctl_rx = rate_x; W.ctl_ry = rate_y;
+    control_update(W);
+    W.ctl_mode = mode; W.ctl_rx = rx; W.ctl_ry = ry;
+    return XDEM_OK;
+}
+
 int oracle_set_control(void* h, int mode, int reset_rattlers, double rattler_scale,
                        double stress, double rate_x, double rate_y, double mass,
                        double max_rate) {

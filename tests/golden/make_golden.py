@@ -7,7 +7,7 @@ files.  Run it in a container where /root/reference is present:
 
     make -C oracle && python tests/golden/make_golden.py [group ...]
 
-with groups world, control, history, checkpoint, pairs (default: all).
+with groups world, control, history, checkpoint, quasistatic, pairs (default: all).
 
 Fixtures
   world_<case>.npz  initial state (+ optional history) and the reference's
@@ -26,6 +26,10 @@ Fixtures
                     after 4 steps, and a world rebuilt from it by the
                     reference's stream constructors, dumped after each of 3
                     further steps
+  quasistatic_<case>.npz  the quasi-static compression loop of
+                    mainIsoElpsPackTestEntropy.cpp:140-180 for 40 steps: the
+                    box and whether it moved after every step, full dumps at
+                    a few steps
   pairs.npz         calculateContactGeometry of the reference on random
                     ellipse and circle poses (src/EllipticParticle.cpp:16-285,
                     src/CircularParticle.cpp:67-92)
@@ -155,6 +159,44 @@ def run_checkpoint(name, tmp):
     print(f"checkpoint {name}: {len(out['file_ckpt'])} bytes of text")
 
 
+QUASI_STEPS = 40
+QUASI_DUMP = (1, 12, 13, 14, 40)
+
+
+def quasi_state(name):
+    """The ellipse packing with the entropy driver's compression model
+    (restitution 0.8, global damping 0.5, friction 0.5); "loose": box and
+    positions scaled by 1.3 (no contacts: elastic energy 0), so that the
+    criterion compresses every 12 steps."""
+    s = helpers.state("pb_ellipse")
+    s.cfg.restitution = 0.8
+    s.cfg.mu = 0.5
+    s.cfg.global_damping = 0.5
+    if name == "loose":
+        f = 1.3
+        s.x = s.x * f
+        s.y = s.y * f
+        s.cfg.xmin *= f; s.cfg.xmax *= f; s.cfg.ymin *= f; s.cfg.ymax *= f
+    return s
+
+
+def run_quasistatic(name, tmp):
+    s = quasi_state(name)
+    sp = os.path.join(tmp, f"qs_{name}.xst")
+    write_state(sp, s)
+    steps = po.run_reference_quasistatic(sp, os.path.join(tmp, f"qs_{name}.bin"), QUASI_STEPS)
+    out = {"in_" + k: v for k, v in state_arrays(s).items()}
+    out["box"] = np.stack([d["box"] for d in steps])
+    out["moved"] = np.array([d["moved"] for d in steps])
+    for st in QUASI_DUMP:
+        for k, v in steps[st - 1].items():
+            if k not in ("step", "box", "moved"):
+                out[f"s{st}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, f"quasistatic_{name}.npz"), **out)
+    print(f"quasistatic {name}: n={s.n}, boundary moved at steps "
+          f"{[k + 1 for k, m in enumerate(out['moved']) if m]}")
+
+
 def run_pairs(tmp, m=3000, seed=42):
     rng = np.random.default_rng(seed)
     rows = np.zeros((m, 12))
@@ -181,7 +223,8 @@ def main():
     if not po.have_reference():
         sys.exit("oracle/_ref/ref_harness missing: run `make -C oracle` where the "
                  "reference sources are present")
-    groups = set(sys.argv[1:]) or {"world", "control", "history", "checkpoint", "pairs"}
+    groups = set(sys.argv[1:]) or {"world", "control", "history", "checkpoint", "quasistatic",
+                                   "pairs"}
     with tempfile.TemporaryDirectory() as tmp:
         if "world" in groups:
             for c in CASES:
@@ -195,6 +238,9 @@ def main():
         if "checkpoint" in groups:
             for c in CHECKPOINT_CASES:
                 run_checkpoint(c, tmp)
+        if "quasistatic" in groups:
+            for c in ("loose", "dense"):
+                run_quasistatic(c, tmp)
         if "pairs" in groups:
             run_pairs(tmp)
 

@@ -93,3 +93,26 @@ def load_checkpoint(name):
         pre = f"r{st}_"
         steps[st] = {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
     return s, z["file_ckpt"].tobytes().decode(), steps
+
+
+QUASI_CASES = ("loose", "dense")
+QUASI_STEPS = 40
+QUASI_DUMP = (1, 12, 13, 14, 40)
+
+
+def load_quasistatic(name):
+    """(state, box per step (QUASI_STEPS x 10), moved per step, {step: dump})
+    of the reference's quasi-static loop (tests/golden/make_golden.py:run_quasistatic)."""
+    z = np.load(os.path.join(GOLDEN, f"quasistatic_{name}.npz"))
+    cfgv = z["in_cfg"]
+    cfg = WorldConfig(world_kind=int(cfgv[0]), **{k: float(v) for k, v in zip(_CFG_KEYS, cfgv[1:])})
+    f = lambda k: z["in_" + k].copy()
+    s = State(cfg, *(f(k) for k in ("kind", "up_level", "flags", "r", "aspect", "mass",
+                                     "inertia", "x", "y", "theta", "vx", "vy", "omega",
+                                     "clump_bx", "clump_by", "ci", "cj", "xi_x", "xi_y",
+                                     "n_x", "n_y", "damping")))
+    steps = {}
+    for st in QUASI_DUMP:
+        pre = f"s{st}_"
+        steps[st] = {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}
+    return s, z["box"], z["moved"], steps

@@ -1,0 +1,65 @@
+"""The quasi-static compression loop (mainIsoElpsPackTestEntropy.cpp:140-180)
+on the GPU engine against the same loop on the CPU oracle, which
+tests/test_quasistatic.py pins to the real reference: the boundary moves
+at the same steps, the box agrees to 1e-12 and positions to 1e-10 over 40
+steps; and the whole entropy-driver protocol at a small size."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.golden_io import QUASI_CASES, QUASI_STEPS, load_quasistatic
+from tests.test_quasistatic import run_loop
+from xdem_amd import prepare as pp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", QUASI_CASES)
+def test_engine_loop_matches_oracle(name):
+    from xdem_amd.engine import Engine
+    s, _, _, _ = load_quasistatic(name)
+    print(f"[gpu] quasi-static loop {name}", flush=True)
+    eng, orc = Engine(s), po.OracleWorld(s)
+    eng.set_control(**pp.quasi_static_control())
+    orc.set_control(**pp.quasi_static_control())
+    me = run_loop(eng, s.n, QUASI_STEPS)
+    mo = run_loop(orc, s.n, QUASI_STEPS)
+    assert me == mo
+    be, bo = eng.box()[0], orc.box()
+    assert np.all(np.abs(be[:4] - bo[:4]) <= 1e-12 * np.abs(bo[:4]))
+    pe, po_ = eng.particles(), orc.particles()
+    for k in ("x", "y", "theta"):
+        assert float(np.max(np.abs(pe[k] - po_[k]))) <= 1e-10, k
+    ce, co = eng.contacts(), orc.contacts()
+    assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"])
+    eng.close()
+
+
+def test_apply_strain_matches_oracle():
+    from xdem_amd.engine import Engine
+    s, _, _, _ = load_quasistatic("dense")
+    print("[gpu] apply_strain", flush=True)
+    eng, orc = Engine(s), po.OracleWorld(s)
+    for w in (eng, orc):
+        w.step(3)
+        w.apply_strain(-0.5, -0.3)
+        w.step(2)
+    be, bo = eng.box()[0], orc.box()
+    assert np.all(np.abs(be[:4] - bo[:4]) <= 1e-13 * np.abs(bo[:4]))
+    assert be[1] < s.cfg.xmax and be[3] < s.cfg.ymax
+    pe, po_ = eng.particles(), orc.particles()
+    for k in ("x", "y", "theta"):
+        assert float(np.max(np.abs(pe[k] - po_[k]))) <= 1e-11, k
+    ce, co = eng.contacts(), orc.contacts()
+    assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"])
+    eng.close()
+
+
+def test_entropy_protocol_small():
+    print("[gpu] entropy-driver protocol n=300", flush=True)
+    s, moves = pp.prepare_packing_quasi_static(300, 1.5, friction=0.5, solid_fraction=0.8,
+                                               steps=300, check_every=100)
+    assert moves >= 1
+    assert s.cfg.xmax == -s.cfg.xmin and s.cfg.ymax == -s.cfg.ymin
+    for k in ("x", "y", "vx", "vy", "omega"):
+        assert np.all(np.isfinite(getattr(s, k)))

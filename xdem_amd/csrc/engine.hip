@@ -2937,6 +2937,53 @@ int xdem_set_control(xdem_engine* E, int packing, const xdem_control* c) {
     return XDEM_OK;
 }
 
+int xdem_apply_strain(xdem_engine* E, int packing, double strain_rate_x,
+                      double strain_rate_y) {
+    if (!E || !E->have_particles) return fail(E, XDEM_E_INVALID, "set particles first");
+    if (packing < 0 || packing >= E->npacks) return fail(E, XDEM_E_INVALID, "bad packing");
+    if (!std::isfinite(strain_rate_x) || !std::isfinite(strain_rate_y))
+        return fail(E, XDEM_E_NONFINITE, "non-finite strain rate");
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    std::vector<Pack> saved(E->npacks);
+    HIPCHK(hipMemcpy(saved.data(), E->packs_d, sizeof(Pack) * E->npacks, hipMemcpyDeviceToHost));
+    if (saved[packing].world != XDEM_WORLD_PERIODIC)
+        return fail(E, XDEM_E_INVALID, "boundary control needs a periodic packing");
+    // one boundary update of the strain mode on this packing alone: the
+    // other packings and the rattler reset are switched off for the launch
+    std::vector<Pack> once = saved;
+    for (int k = 0; k < E->npacks; ++k) { once[k].ctl_mode = 0; once[k].ctl_reset = 0; }
+    once[packing].ctl_mode = XDEM_CONTROL_STRAIN;
+    once[packing].ctl_rx = strain_rate_x;
+    once[packing].ctl_ry = strain_rate_y;
+    HIPCHK(hipMemcpy(E->packs_d, once.data(), sizeof(Pack) * E->npacks, hipMemcpyHostToDevice));
+    int* rb_next = E->rebuild_d + (E->steps & 1);    // read by the next step
+    hipLaunchKernelGGL(k_control, dim3(E->npacks), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
+                       E->stress_d, rb_next, E->err_d);
+    hipLaunchKernelGGL(k_ctl_particles, dim3(grid_of(E->n)), dim3(kBlock), 0, E->stream, E->P,
+                       E->packs_d, E->err_d);
+    // the next step rebuilds the pair list: without boundary control the
+    // steps do not track the displacement bound that the affine rescaling
+    // would need
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, E->stream, rb_next, 1);
+    HIPCHK(hipGetLastError());
+    if ((rc = xdem_sync(E))) return rc;
+    std::vector<Pack> after(E->npacks);
+    HIPCHK(hipMemcpy(after.data(), E->packs_d, sizeof(Pack) * E->npacks, hipMemcpyDeviceToHost));
+    for (int k = 0; k < E->npacks; ++k) {
+        // keep the new box, grids and list bookkeeping; restore the control
+        // settings and the stress of the last control step
+        after[k].ctl_mode = saved[k].ctl_mode;
+        after[k].ctl_reset = saved[k].ctl_reset;
+        after[k].ctl_rx = saved[k].ctl_rx;
+        after[k].ctl_ry = saved[k].ctl_ry;
+        for (int q = 0; q < 4; ++q) after[k].tstress[q] = saved[k].tstress[q];
+        if (k != packing) after[k] = saved[k];
+    }
+    HIPCHK(hipMemcpy(E->packs_d, after.data(), sizeof(Pack) * E->npacks, hipMemcpyHostToDevice));
+    return XDEM_OK;
+}
+
 int xdem_get_box(xdem_engine* E, double* out) {
     int rc = xdem_sync(E);
     if (rc) return rc;

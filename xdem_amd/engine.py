@@ -28,7 +28,7 @@ SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles
            "xdem_get_forces", "xdem_get_contacts", "xdem_get_contact_branches",
            "xdem_get_stress", "xdem_get_boundary",
            "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
-           "xdem_set_control", "xdem_get_box", "xdem_get_observables",
+           "xdem_set_control", "xdem_apply_strain", "xdem_get_box", "xdem_get_observables",
            "xdem_history_open", "xdem_history_write", "xdem_history_close",
            "xdem_set_mlp", "xdem_get_mlp", "xdem_mlp_init", "xdem_log_roots",
            "xdem_log_count", "xdem_log_fetch", "xdem_mlp_train", "xdem_mlp_grad",
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.xdem_get_traffic.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.xdem_set_control.argtypes = [vp, C.c_int, C.POINTER(XdemControl)]
+    L.xdem_apply_strain.argtypes = [vp, C.c_int, C.c_double, C.c_double]
     L.xdem_get_box.argtypes = [vp, _D]
     L.xdem_get_observables.argtypes = [vp, _D]
     L.xdem_history_open.argtypes = [vp, C.c_int, C.c_char_p]
@@ -250,6 +251,11 @@ class Engine:
                         float(stress), float(rate_x), float(rate_y), float(boundary_mass),
                         float(max_strain_rate))
         self._check(self.L.xdem_set_control(self.h, int(packing), C.byref(c)))
+
+    def apply_strain(self, rate_x: float, rate_y: float, packing: int = 0):
+        """One strain-control boundary update now, between steps
+        (xdem_apply_strain)."""
+        self._check(self.L.xdem_apply_strain(self.h, int(packing), float(rate_x), float(rate_y)))
 
     def box(self):
         """Per packing: xmin, xmax, ymin, ymax, boundary velocity x, y and the

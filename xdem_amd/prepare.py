@@ -86,9 +86,12 @@ def reference_draws(n: int, aspect_ratio: float, seed: int = 0, half_box: float 
 
 
 def reference_generation(n: int, aspect_ratio: float, seed: int = 0, half_box: float = 100.0,
-                         solid_fraction: float = 0.8, density: float = 10.0) -> State:
+                         solid_fraction: float = 0.8, density: float = 10.0, dt: float = 1e-4,
+                         restitution: float = 0.95) -> State:
     """Stage 1: the circles of mainIsoElpsPack_WCCM.cpp:46-73, in a box scaled
-    to the initial solid fraction."""
+    to the initial solid fraction (mainIsoElpsPackTestEntropy.cpp:62-86 draws
+    the same numbers with density 1, dt 5e-5, restitution 1 and the solid
+    fraction of its parameter file)."""
     _, r, x, y, th = reference_draws(n, aspect_ratio, seed, half_box)
     kind = np.full(n, KIND_CIRCLE, np.int32)
     aspect = np.ones(n)
@@ -104,7 +107,7 @@ def reference_generation(n: int, aspect_ratio: float, seed: int = 0, half_box: f
     y = y * scale
     xmin *= scale; xmax *= scale; ymin *= scale; ymax *= scale   # scaleBoundarySize
     cfg = WorldConfig(world_kind=WORLD_PERIODIC, xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax,
-                      dt=1e-4, kn=1e7, kt=1e7, restitution=0.95, mu=0.0)
+                      dt=dt, kn=1e7, kt=1e7, restitution=restitution, mu=0.0)
     z = np.zeros(n)
     return State(cfg, kind, np.arange(n, dtype=np.int32), np.zeros(n, np.int32), r, aspect,
                  mass, inertia, x, y, th, z.copy(), z.copy(), z.copy(), z.copy(), z.copy())
@@ -155,15 +158,20 @@ def _read_back(eng, s: State) -> State:
 
 
 def relax(s: State, max_steps: int = 50_000, check_every: int = 100, min_contacts: int = 20,
-          log=None) -> tuple[State, int]:
+          rattler_scale: float | None = None, log=None) -> tuple[State, int]:
     """Stage 2 (mainIsoElpsPack_WCCM.cpp:74-91): steps with
     resetVelocityOfRattlers until fewer than `min_contacts` contacts remain,
-    tested every `check_every` steps.  Returns the relaxed state and the
-    steps taken."""
+    tested every `check_every` steps.  With `rattler_scale` the rattlers'
+    velocities are scaled by it after every step instead of reset
+    (mainIsoElpsPackTestEntropy.cpp:92-102: 0.5, fewer than 100 contacts).
+    Returns the relaxed state and the steps taken."""
     from .engine import Engine
     eng = Engine(s)
     try:
-        eng.set_control(**relax_control())
+        if rattler_scale is None:
+            eng.set_control(**relax_control())
+        else:
+            eng.set_control(CONTROL_NONE, rattler_scale=float(rattler_scale))
         steps = 0
         while steps < max_steps:
             k = min(check_every, max_steps - steps)
@@ -220,3 +228,89 @@ def prepare_packing(n: int, aspect_ratio: float, pressure: float, friction: floa
                     min_steps=compress_min_steps, max_steps=compress_max_steps,
                     check_every=check_every, log=log)
     return s
+
+
+class QuasiStaticCriterion:
+    """The decision of the quasi-static compression loop of
+    mainIsoElpsPackTestEntropy.cpp:151-177, evaluated after every step from
+    the elastic energy per contact, the number of contacts and the kinetic
+    energy per non-rattler particle: compress when the packing is loose and
+    unstressed, or stressed, well connected and nearly at rest; after a
+    compression, wait more than 10 steps.  Calling it returns True when the
+    boundary is to take one strain update (-0.2, -0.2)."""
+
+    STRAIN_RATE = -0.2
+
+    def __init__(self, n: int):
+        self.crit = int(n * 2.5)          # critCntNum (an int from size() * 2.5)
+        self.count = 0
+        self.count0 = 0
+        self.go = True
+
+    def __call__(self, elastic_per_contact: float, contacts: int, ke_nonrattler: float) -> bool:
+        el, nc = elastic_per_contact, int(contacts)
+        if ((el < 1e-6 and nc < self.crit and self.go)
+                or (el >= 1e-6 and nc >= self.crit and ke_nonrattler < 5e-2 and self.go)):
+            if self.count == 1:
+                self.count = 0
+                self.count0 = 0
+            self.count += 1
+            self.go = False
+            return True
+        self.count0 += 1
+        if self.count0 > 10:
+            self.go = True
+        return False
+
+
+def quasi_static_control() -> dict:
+    """set_control arguments of the quasi-static loop: no boundary control
+    inside the steps, rattlers' velocities scaled by 0.95 after every step."""
+    return dict(mode=CONTROL_NONE, rattler_scale=0.95)
+
+
+def quasi_static_compress(s: State, steps: int, log=None) -> tuple[State, int]:
+    """The compression loop of mainIsoElpsPackTestEntropy.cpp:140-180 on the
+    engine for `steps` steps (the driver's loop has no end): after every step
+    the criterion reads the step's observables (xdem_get_observables) and may
+    move the boundary by one strain update (xdem_apply_strain).  `s` is the
+    packing of ellipses with its contact model (the driver: restitution 0.8,
+    global damping 0.5; ellipses_from_circles).  Returns the state and the
+    number of boundary updates."""
+    from .engine import Engine
+    eng = Engine(s)
+    try:
+        eng.set_control(**quasi_static_control())
+        crit = QuasiStaticCriterion(s.n)
+        moves = 0
+        for k in range(steps):
+            eng.step(1)
+            ob = eng.observables()[0]
+            if crit(float(ob[6]), int(ob[1]), float(ob[4])):
+                eng.apply_strain(crit.STRAIN_RATE, crit.STRAIN_RATE)
+                moves += 1
+                if log:
+                    log(f"[quasi-static] step {k + 1}: boundary moved ({moves}), "
+                        f"{int(ob[1])} contacts")
+        return _read_back(eng, s), moves
+    finally:
+        eng.close()
+
+
+def prepare_packing_quasi_static(n: int, aspect_ratio: float, friction: float,
+                                 solid_fraction: float, steps: int, seed: int = 0,
+                                 relax_max_steps: int = 50_000, check_every: int = 100,
+                                 log=None) -> tuple[State, int]:
+    """The protocol of mainIsoElpsPackTestEntropy.cpp:46-180 on the engine:
+    generation (density 1, dt 5e-5, restitution 1), relaxation with rattler
+    velocities halved until fewer than 100 contacts remain, then `steps`
+    steps of the quasi-static compression of the ellipses (restitution 0.8,
+    global damping 0.5)."""
+    s = reference_generation(n, aspect_ratio, seed, solid_fraction=solid_fraction, density=1.0,
+                             dt=5e-5, restitution=1.0)
+    s, _ = relax(s, max_steps=relax_max_steps, check_every=check_every, min_contacts=100,
+                 rattler_scale=0.5, log=log)
+    e = ellipses_from_circles(s, aspect_ratio, friction, restitution=0.8, density=1.0,
+                              global_damping=0.5)
+    return quasi_static_compress(e, steps, log=log)
+

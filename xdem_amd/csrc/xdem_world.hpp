@@ -211,6 +211,13 @@ public:
     void stressControlUniformStrainLimitEachStep(double stress) {    // :213-237
         ctl_.mode = XDEM_CONTROL_STRESS_LIMITED; ctl_.stress = stress;
     }
+    // one strain update of the box now, between steps, as a driver calls it
+    // after reading the observables (mainIsoElpsPackTestEntropy.cpp:171)
+    void updatePeriodicBoundary_strainControl(double rateX, double rateY) {  // :124-135
+        upload();
+        check(xdem_apply_strain(e_, 0, rateX, rateY));
+        dirty_ = true;
+    }
     // xmin, xmax, ymin, ymax, boundary velocity x, y, control stress (4)
     std::vector<double> box() {
         upload();

@@ -205,6 +205,25 @@ def cpu_baseline(s, steps: int, threads: int):
             "sample": f"{steps} steps of the same {s.n}-particle packing (after 1 warm-up step)"}
 
 
+def copy_bandwidth(reps: int = 10, n: int = 64 << 20) -> float:
+    """Achieved HBM bandwidth of a plain streaming copy on this device
+    (b = 2a over n doubles: 16n bytes moved per launch), GB/s, timed with
+    HIP events: the practical ceiling next to the published 8 TB/s."""
+    import torch
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.mul(a, 2.0, out=b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.mul(a, 2.0, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 16.0 * n * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    return gbs
+
+
 def copy_probe(reps: int = 5, n: int = 64 << 20):
     """A plain streaming kernel of known traffic (b = 2a over n doubles:
     8n bytes read, 8n bytes written per launch).  Run under rocprofv3 --pmc,
@@ -228,7 +247,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--relax", type=int, default=6000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=200)
     ap.add_argument("--fp64-only", action="store_true")
@@ -283,6 +302,7 @@ def main():
     eng.set_timing(False)
     eng.close()
     ms_per_step = elapsed * 1e3 / args.steps
+    copy_gbs = copy_bandwidth()
     value = s.n * args.steps / elapsed
     kern_avg_ms = kern_ms / max(launches, 1)
     # k_contact's algorithmic bytes: each particle's pose, velocity and shape
@@ -327,7 +347,8 @@ def main():
                      "kernel": "k_contact", "kernel_avg_ms": kern_avg_ms,
                      "algorithmic_bytes_per_launch": kern_bytes,
                      "step_algorithmic_bytes": bytes_step,
-                     "step_hbm_frac": bytes_step * args.steps / elapsed / (HBM_PEAK_GBS * 1e9)},
+                     "step_hbm_frac": bytes_step * args.steps / elapsed / (HBM_PEAK_GBS * 1e9),
+                     "measured_copy_gbs": copy_gbs},
         "modes": modes,
         "ensemble": ensemble,
     }

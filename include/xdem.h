@@ -137,6 +137,13 @@ int xdem_sync(xdem_engine* e);
 int xdem_get_particles(xdem_engine* e, double* x, double* y, double* theta,
                        double* vx, double* vy, double* omega);
 
+/* Branch vectors of clump sub-particles (sub centre minus clump centre) as
+ * they stand after the last step; 0 for other particles.  They turn with
+ * the clump every step (GluedParticle::_branchVectors,
+ * src/GluedParticle.cpp:72, 116-120), so a restart needs them with the
+ * particle state.  Either pointer may be NULL. */
+int xdem_get_clump_branches(xdem_engine* e, double* bx, double* by);
+
 /* Per-particle force and torque of the last step
  * (BaseWorld::_totalForce/_totalTorque, src/BaseWorld.cpp:17-57) and the
  * per-particle contact count (_numContactPerParticle). */

@@ -47,6 +47,7 @@ def lib():
         L.oracle_step.argtypes = [C.c_void_p, C.c_int]
         L.oracle_get_particles.argtypes = [C.c_void_p] + [_D] * 6
         L.oracle_get_forces.argtypes = [C.c_void_p, _D, _D, _D, _I]
+        L.oracle_get_clump_branches.argtypes = [C.c_void_p, _D, _D]
         L.oracle_num_contacts.argtypes = [C.c_void_p]
         L.oracle_get_contacts.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _I, _I] + [_D] * 13
         L.oracle_get_branches.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)] + [_D] * 6
@@ -194,6 +195,9 @@ class OracleWorld:
                                c["n_y"], c["damping"])
         b = self.boundary()
         out.cfg.total_shift, out.cfg.total_strain = float(b[0]), float(b[2])
+        bx, by = np.zeros(out.n), np.zeros(out.n)
+        lib().oracle_get_clump_branches(self.h, bx, by)
+        out.clump_bx, out.clump_by = bx, by
         return out
 
 

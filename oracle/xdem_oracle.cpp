@@ -914,6 +914,12 @@ int oracle_get_particles(void* h, double* x, double* y, double* th, double* vx,
     return XDEM_OK;
 }
 
+int oracle_get_clump_branches(void* h, double* bx, double* by) {
+    World& W = *(World*)h;
+    for (int i = 0; i < W.n; ++i) { bx[i] = W.cbx[i]; by[i] = W.cby[i]; }
+    return XDEM_OK;
+}
+
 int oracle_get_forces(void* h, double* fx, double* fy, double* tq, int* nct) {
     World& W = *(World*)h;
     for (int i = 0; i < W.n; ++i) {

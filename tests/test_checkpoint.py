@@ -136,3 +136,26 @@ def test_writer_reader_round_trip(tmp_path):
     for k in ("xi_x", "xi_y", "n_x", "n_y", "damping"):
         a, b = getattr(back, k), getattr(st, k)
         assert np.all(np.abs(a - b) <= 5.01e-9 * np.abs(b) + 1e-300), k
+
+
+@pytest.mark.parametrize("name", ["pb_clumps", "le_poly"])
+def test_oracle_binary_state_restart_is_exact(name, tmp_path):
+    """The exact binary state (state.write_state of an exported world,
+    clump branch vectors as the clumps have turned them included) restarts
+    the oracle bit for bit: 9 further steps equal the uninterrupted run."""
+    from tests import helpers
+    from xdem_amd.state import read_state, write_state
+    s = helpers.state(name)
+    a = po.OracleWorld(s)
+    a.step(12)
+    path = str(tmp_path / "s.xdst")
+    write_state(path, a.export_state(s))
+    b = po.OracleWorld(read_state(path))
+    a.step(9)
+    b.step(9)
+    pa, pb = a.particles(), b.particles()
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
+    ca, cb = a.contacts(), b.contacts()
+    for k in ca:
+        assert np.array_equal(ca[k], cb[k]), k

@@ -36,7 +36,7 @@ import math
 
 import numpy as np
 
-from .state import KIND_CIRCLE, KIND_CLUMP, KIND_ELLIPSE, State, WorldConfig
+from .state import KIND_CIRCLE, KIND_CLUMP, KIND_ELLIPSE, State, WorldConfig, write_state
 
 _PARTICLE = "%7d\t" + "\t".join(["%.8e"] * 12) + "\t%4d\n"
 _CONTACT = "\t%7d\t" + "\t".join(["%.8e"] * 25) + "\n"
@@ -81,14 +81,18 @@ def write_all_particles(path: str, s: State, contacts: dict, branches: dict,
                 f.write(_CONTACT % ((int(cj[c]),) + tuple(float(a[c]) for a in cols)))
 
 
-def engine_checkpoint(eng, path: str, density: float = 1.0) -> None:
-    """print2File_allParticles of a one-packing Engine after its last step."""
+def engine_checkpoint(eng, path: str, density: float = 1.0, sidecar: str | None = None) -> None:
+    """print2File_allParticles of a one-packing Engine after its last step;
+    with `sidecar`, also the exact binary state there (Engine.export_states,
+    state.write_state), from which a restart continues bit for bit."""
     if len(eng.states) != 1:
         raise ValueError("one packing per text checkpoint")
     s = eng.states[0].copy()
     for k, v in eng.particles().items():
         setattr(s, k, v)
     write_all_particles(path, s, eng.contacts(), eng.contact_branches(), density)
+    if sidecar is not None:
+        write_state(sidecar, eng.export_states()[0])
 
 
 def read_all_particles(path: str, cfg: WorldConfig, aspect_ratio=None,

@@ -2767,6 +2767,13 @@ int xdem_get_particles(xdem_engine* E, double* x, double* y, double* theta, doub
     return XDEM_OK;
 }
 
+int xdem_get_clump_branches(xdem_engine* E, double* bx, double* by) {
+    int rc = xdem_sync(E);
+    if (rc) return rc;
+    if ((rc = fetch_ext(E, bx, E->P.cbx)) || (rc = fetch_ext(E, by, E->P.cby))) return rc;
+    return XDEM_OK;
+}
+
 int xdem_get_forces(xdem_engine* E, double* fx, double* fy, double* torque, int* num_contacts) {
     int rc = xdem_sync(E);
     if (rc) return rc;

@@ -25,7 +25,8 @@ ERRORS = {-1: "XDEM_E_INVALID", -2: "XDEM_E_CAPACITY", -3: "XDEM_E_UNSUPPORTED_P
 # exported entry points of include/xdem.h
 SYMBOLS = ("xdem_create", "xdem_destroy", "xdem_last_error", "xdem_set_particles",
            "xdem_set_contacts", "xdem_step", "xdem_sync", "xdem_get_particles",
-           "xdem_get_forces", "xdem_get_contacts", "xdem_get_contact_branches",
+           "xdem_get_clump_branches", "xdem_get_forces", "xdem_get_contacts",
+           "xdem_get_contact_branches",
            "xdem_get_stress", "xdem_get_boundary",
            "xdem_get_counters", "xdem_set_timing", "xdem_get_timing", "xdem_get_traffic",
            "xdem_set_control", "xdem_apply_strain", "xdem_get_box", "xdem_get_observables",
@@ -80,6 +81,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xdem_sync.argtypes = [vp]
     L.xdem_get_particles.argtypes = [vp] + [_D] * 6
     L.xdem_get_forces.argtypes = [vp, _D, _D, _D, _I]
+    L.xdem_get_clump_branches.argtypes = [vp, _D, _D]
     L.xdem_get_contacts.argtypes = [vp, C.c_int, C.POINTER(C.c_int)] + [vp] * 15
     L.xdem_get_contact_branches.argtypes = [vp, C.c_int, C.POINTER(C.c_int)] + [vp] * 6
     L.xdem_get_stress.argtypes = [vp, _D]
@@ -203,6 +205,13 @@ class Engine:
         self._check(self.L.xdem_get_particles(self.h, *out))
         return dict(zip(("x", "y", "theta", "vx", "vy", "omega"), out))
 
+    def clump_branches(self):
+        """Current branch vectors of clump sub-particles (0 elsewhere),
+        xdem_get_clump_branches."""
+        bx, by = np.zeros(self.n), np.zeros(self.n)
+        self._check(self.L.xdem_get_clump_branches(self.h, bx, by))
+        return bx, by
+
     def forces(self):
         fx, fy, tq = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
         nc = np.zeros(self.n, np.int32)
@@ -223,6 +232,31 @@ class Engine:
         d = dict(zip(names, arrs))
         d["i"], d["j"] = ci, cj
         return d
+
+    def export_states(self):
+        """The current state of every packing as State objects: particles,
+        box (after boundary control), Lees-Edwards shift and strain, and the
+        history (shear overlap, normal, damping constant) of every contact
+        the last step holds, indexed within the packing, and the clump
+        branch vectors as the clumps have turned them.  Written with
+        state.write_state, it is the exact binary sidecar of a text
+        checkpoint: an Engine built from it continues the run bit for bit.
+        Control settings (set_control) and the boundary velocity are not
+        part of it."""
+        p, c, box, b = self.particles(), self.contacts(), self.box(), self.boundary()
+        p["clump_bx"], p["clump_by"] = self.clump_branches()
+        out = []
+        for k, s0 in enumerate(self.states):
+            a, z = int(self.starts[k]), int(self.starts[k + 1])
+            s = s0.copy()
+            for key in ("x", "y", "theta", "vx", "vy", "omega", "clump_bx", "clump_by"):
+                setattr(s, key, p[key][a:z].copy())
+            s.cfg.xmin, s.cfg.xmax, s.cfg.ymin, s.cfg.ymax = (float(v) for v in box[k, :4])
+            s.cfg.total_shift, s.cfg.total_strain = float(b[k, 0]), float(b[k, 2])
+            m = (c["i"] >= a) & (c["i"] < z)
+            out.append(s.with_history(c["i"][m] - a, c["j"][m] - a, c["xi_x"][m], c["xi_y"][m],
+                                      c["n_x"][m], c["n_y"][m], c["damping"][m]))
+        return out
 
     def contact_branches(self):
         """Branch vectors of the contacts() entries, same order

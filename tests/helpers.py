@@ -150,3 +150,32 @@ def pair_lattice_state(poses, spacing=4.0, world_kind=WORLD_PERIODIC, shear_rate
     z = np.zeros(2 * n)
     return State(cfg, kind, np.arange(2 * n, dtype=np.int32), np.zeros(2 * n, np.int32),
                  r, ar, m, inertia, x, y, th, z.copy(), z.copy(), z.copy(), z.copy(), z.copy())
+
+
+def nan_pose_lattice(n=600, seed=31):
+    """pair_lattice_state of n near-touching pairs in the box [0, L) x [-L, 0)
+    with five sites holding the pose for which the reference's arithmetic
+    gives a NaN contact (test_oracle_math.NAN_POSE).  Those sites sit near
+    the pose's own position and are moved by whole units, so both centres
+    keep their binades and the centre difference is the pose's to the bit.
+    Returns (state, site indices)."""
+    from tests.test_oracle_math import NAN_POSE as p
+    poses = near_touching_poses(n, seed=seed)
+    s = pair_lattice_state(poses)
+    side = int(np.ceil(np.sqrt(n)))
+    L = s.cfg.ymax
+    s.y -= L
+    s.cfg.ymin, s.cfg.ymax = -L, 0.0
+    sites = []
+    for m, row in ((10, 11), (11, 11), (12, 12), (13, 13), (11, 13)):
+        k = m + side * row
+        cx, cy = (m + 0.5) * 4.0, (row + 0.5) * 4.0 - L
+        tx, ty = cx - 48.0, cy + 46.0
+        a, b = 2 * k, 2 * k + 1
+        s.x[a], s.y[a], s.x[b], s.y[b] = p[2] + tx, p[3] + ty, p[7] + tx, p[8] + ty
+        s.r[a], s.aspect[a], s.theta[a] = p[0], p[1], p[4]
+        s.r[b], s.aspect[b], s.theta[b] = p[5], p[6], p[9]
+        sites.append(k)
+    from xdem_amd.state import particle_properties
+    s.mass, s.inertia, _ = particle_properties(s.kind, s.r, s.aspect)
+    return s, sites

@@ -360,3 +360,28 @@ def test_cpp_adapter_control_world(tmp_path):
     assert np.array_equal(np.array(out["stress"]), eng.stress()[0])
     assert np.array_equal(np.array(out["box"]), eng.box()[0][:4])
     assert out["solid_fraction"] == obs[8]
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_nonfinite_geometry_repair_among_other_pairs(precision):
+    """The NaN-contact pose at five sites of a lattice of 600 near-touching
+    pairs: the lanes that redo the geometry with ordered eigenvalues share
+    their wavefronts with ordinary contacts and non-contacts.  Contact sets
+    equal the oracle's in every precision mode, forces are finite, and the
+    plain FP64 mode matches the oracle to the parity bars."""
+    s, sites = helpers.nan_pose_lattice()
+    print(f"[gpu] NaN-contact pose among 600 pairs, precision {precision}", flush=True)
+    ref, env = oracle_with_envelope(s)
+    eng = _engine(s, precision=precision)
+    if precision == 2:
+        eng.mlp_init(0)
+    eng.step(1)
+    gpu = _readback(eng)
+    ce, co = gpu["contacts"], ref["contacts"]
+    assert np.array_equal(ce["i"], co["i"]) and np.array_equal(ce["j"], co["j"])
+    assert all(int(2 * k) in set(ce["i"].tolist()) for k in sites)
+    assert np.all(np.isfinite(gpu["forces"]["fx"])) and np.all(np.isfinite(ce["n_x"]))
+    assert eng.counters()["rescues"] >= len(sites)
+    if precision == 0:
+        assert_parity(gpu, ref, env, "repaired pairs among others")
+    eng.close()

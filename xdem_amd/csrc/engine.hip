@@ -1029,17 +1029,25 @@ struct StepCounts {
 // contact list (wavefront-compacted, so the writes below are contiguous),
 // and link[e] points to it.  Every active lane of the wavefront must call
 // this function (it allocates per wavefront once the geometry is known).
+// fix = false: the reference's arithmetic.  When that gives no valid answer
+// (non-finite contact geometry) the lane neither allocates nor writes and
+// returns true: the caller then calls again with fix = true (eigenvalues
+// ordered by |lambda|).  The second call reloads its inputs, so none of the
+// first attempt's values stay live across it (117 instead of 147 VGPRs in
+// the contact kernel, one more wavefront per SIMD).
 template <int PREC>
-__device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, const Pack& K,
+__device__ __forceinline__ bool contact_item(const Parts& P, const Frame& F, const Pack& K,
                                              const Table& old_t, const Table& new_t, int e,
                                              int lst, int found, int i, int j, float guess,
-                                             bool record, uint32_t* err, StepCounts& n) {
+                                             bool record, bool fix, uint32_t* err,
+                                             StepCounts& n) {
     const double xi = P.x[i], yi = P.y[i];
     double ox, oy, ovx;
     image_offset(K, lst, xi, yi, P.x[j], P.y[j], ox, oy, ovx);
     const double xj2 = P.x[j] + ox, yj2 = P.y[j] + oy;
     Geom g;
     int hit;
+    bool retry = false;
     if (P.kind[i] == KIND_CIRCLE) {
         hit = circle_contact(xi, yi, P.r[i], xj2, yj2, P.r[j], g) ? 1 : 0;
     } else {
@@ -1050,37 +1058,35 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
         frame_m(f.kx, f.ky, F.eta[i], f.mxx, f.mxy, f.myy);
         f.b = F.b[i]; f.e2 = F.e2[i]; f.bsq = F.bsq[i];
         const double k2x = F.kx[j], k2y = F.ky[j], e2s = F.e2[j], b2s = F.bsq[j];
-        // attempt 0: the reference's arithmetic; attempt 1 only when that
-        // gives no valid answer (non-finite contact geometry): eigenvalues
-        // ordered by |lambda|
-        hit = 0;
-        for (int attempt = 0; attempt < 2; ++attempt) {
-            ZpmSetup S;
-            hit = zpm_setup_frame(f, k2x, k2y, e2s, b2s, dx, dy, S, attempt == 1);
-            if (!hit) break;
+        ZpmSetup S;
+        hit = zpm_setup_frame(f, k2x, k2y, e2s, b2s, dx, dy, S, fix);
+        if (hit) {
             double q;
-            if (!solve_root<PREC>(S, K, attempt == 0 ? guess : NAN, q, n.newton, n.rescue,
-                                  n.fallback)) {
+            if (!solve_root<PREC>(S, K, fix ? NAN : guess, q, n.newton, n.rescue, n.fallback)) {
                 hit = -2;
-                break;
+            } else {
+                hit = zpm_finish_frame(S, q, xi, yi, xj2, yj2, dx, dy, F.ixx[i], F.ixy[i],
+                                       F.iyy[i], F.axx[i], F.axy[i], F.ayy[i], g);
+                if (hit == 1 && !fix &&
+                    !(isfinite(g.overlap) && isfinite(g.nx) && isfinite(g.ny) &&
+                      isfinite(g.px) && isfinite(g.py))) {
+                    if (K.strict) {
+                        hit = -2;
+                    } else {
+                        ++n.rescue;
+                        retry = true;
+                        hit = 0;
+                    }
+                }
             }
-            hit = zpm_finish_frame(S, q, xi, yi, xj2, yj2, dx, dy, F.ixx[i], F.ixy[i],
-                                   F.iyy[i], F.axx[i], F.axy[i], F.ayy[i], g);
-            if (hit == 1 && attempt == 0 &&
-                !(isfinite(g.overlap) && isfinite(g.nx) && isfinite(g.ny) &&
-                  isfinite(g.px) && isfinite(g.py))) {
-                if (K.strict) { hit = -2; break; }
-                ++n.rescue;
-                continue;
-            }
-            break;
         }
         if (hit == -2) { raise_err(err, ERR_NEWTON_CAP); hit = 0; }
         if (hit < 0) { raise_err(err, ERR_ZERO_NORMAL); hit = 0; }
     }
     const int ct = alloc_wave(new_t.ccounter, new_t.cseg_cap, hit != 0, err);
+    if (retry) return true;
     new_t.link[e] = ct;
-    if (ct < 0) return;
+    if (ct < 0) return false;
     const bool le = K.world == WORLD_LE;
     double xix, xiy, c, fnx, fny, fsx, fsy, t1, t2, fx, fy;
     if (PREC == kMixed) {
@@ -1229,11 +1235,13 @@ __device__ __forceinline__ void contact_item(const Parts& P, const Frame& F, con
         d.b1x = g.b1x; d.b1y = g.b1y; d.b2x = g.b2x; d.b2y = g.b2y;
         new_t.det[ct] = d;
     }
+    return false;
 }
 
-// One lane per work item.
+// One lane per work item, at most 128 VGPRs (4 wavefronts per SIMD): every
+// build fits without spills.
 template <int PREC, bool ONE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 4)
 k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table new_t,
           WorkList W, const int* __restrict__ record_flag, uint32_t* err, Counters* cnt) {
     __shared__ int s_pre[kSeg + 1];
@@ -1269,9 +1277,13 @@ k_contact(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table n
         const int lst = (int)(ux >> kLstShift);
         const int i = it.z, j = it.w;
         const float guess = PREC == kNN ? W.guess[at] : 0.0f;
-        const Pack K = packs[ONE ? 0 : P.pack[i]];
-        contact_item<PREC>(P, F, K, old_t, new_t, e, lst, item_found(it.y), i, j, guess,
-                           record || item_risky(it.y), err, n);
+        // a reference: each field is loaded where it is used
+        const Pack& K = packs[ONE ? 0 : P.pack[i]];
+        const bool rec = record || item_risky(it.y);
+        if (contact_item<PREC>(P, F, K, old_t, new_t, e, lst, item_found(it.y), i, j, guess, rec,
+                               false, err, n))
+            contact_item<PREC>(P, F, K, old_t, new_t, e, lst, item_found(it.y), i, j, guess, rec,
+                               true, err, n);
     }
     Counters* cs = counter_slot(cnt);
     count_add(&cs->newton, n.newton);

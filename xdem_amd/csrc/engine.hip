@@ -562,56 +562,68 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
             const int rci = P.vrc_of[i];
             const int row_i = rci >> 16, col_i = rci & 0xffff;
             int ncand[3] = {0, 0, 0};
-            for (int jx0 = -1; jx0 <= 1; ++jx0) {
-                int jx = col_i + jx0;
+            // The 3x3 stencil: bin ids and image codes, then the nine bin
+            // counts (independent loads), then the candidates of all nine
+            // bins as one sequence in groups of kVlBatch, every load of a
+            // group issued before any of its results is used.
+            int bin[9], imgc[9], beg[10];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                int jx = col_i + (q / 3 - 1);
                 if (jx == -1) jx = K.vnx - 1;
                 else if (jx == K.vnx) jx = 0;
-                for (int jy0 = -1; jy0 <= 1; ++jy0) {
-                    const int row = row_i + jy0;
-                    int img = 0;
-                    if (K.world == WORLD_LE) img = row == 0 ? 2 : (row == K.vny + 1 ? 1 : 0);
-                    const int b = K.bin_base + row * K.vnx + jx;
-                    const int nb = min(B.count[b], B.cap);
-                    const int* lst = B.list + (size_t)b * B.cap;
-                    // candidates in groups of kVlBatch: every load of a group is
-                    // issued before any of its results is used
-                    for (int t0 = 0; t0 < nb; t0 += kVlBatch) {
-                        int jj[kVlBatch], ej[kVlBatch], uj[kVlBatch];
-                        int8_t fj[kVlBatch], kj[kVlBatch];
-                        int2 cj[kVlBatch];
-                        double xj[kVlBatch], yj[kVlBatch], rj[kVlBatch];
+                const int row = row_i + (q % 3 - 1);
+                imgc[q] = K.world == WORLD_LE ? (row == 0 ? 2 : (row == K.vny + 1 ? 1 : 0)) : 0;
+                bin[q] = K.bin_base + row * K.vnx + jx;
+            }
+            beg[0] = 0;
 #pragma unroll
-                        for (int u = 0; u < kVlBatch; ++u)
-                            jj[u] = t0 + u < nb ? lst[t0 + u] : i;
+            for (int q = 0; q < 9; ++q) beg[q + 1] = min(B.count[bin[q]], B.cap);
 #pragma unroll
-                        for (int u = 0; u < kVlBatch; ++u) {
-                            const int j = jj[u];
-                            ej[u] = P.eid[j]; uj[u] = P.up[j]; fj[u] = P.flags[j];
-                            kj[u] = P.kind[j]; cj[u] = P.cell[j];
-                            xj[u] = P.x[j]; yj[u] = P.y[j]; rj[u] = P.r[j];
-                        }
+            for (int q = 0; q < 9; ++q) beg[q + 1] += beg[q];
+            const int tot = beg[9];
+            for (int t0 = 0; t0 < tot; t0 += kVlBatch) {
+                int jj[kVlBatch], im[kVlBatch], ej[kVlBatch], uj[kVlBatch];
+                int8_t fj[kVlBatch], kj[kVlBatch];
+                int2 cj[kVlBatch];
+                double xj[kVlBatch], yj[kVlBatch], rj[kVlBatch];
 #pragma unroll
-                        for (int u = 0; u < kVlBatch; ++u) {
-                            if (t0 + u >= nb) break;
-                            const int j = jj[u];
-                            if (eidi >= ej[u]) continue;
-                            if (K.world == WORLD_PERIODIC && (fj[u] & SKIP_CONTACT_DETECTION)) continue;
-                            if (upi == uj[u]) continue;
-                            // the reference's own candidate lists (its exit checks)
-                            const int cl = cell_list_of(K, ci, cj[u]);
-                            if (cl >= 0) {
-                                ncand[cl]++;
-                                if (ki != KIND_CLUMP && kj[u] != ki) raise_err(err, ERR_UNSUPPORTED);
-                            }
-                            if (ki == KIND_CLUMP || kj[u] != ki) continue;
-                            double ox, oy, ovx;
-                            image_offset(K, img, xi, yi, xj[u], yj[u], ox, oy, ovx);
-                            const double qx = xi - (xj[u] + ox), qy = yi - (yj[u] + oy);
-                            if ((ri + rj[u]) > sqrt(qx * qx + qy * qy)) {
-                                if (nl < kMaxList) s_list[nl][tid] = j;
-                                ++nl;
-                            }
-                        }
+                for (int u = 0; u < kVlBatch; ++u) {
+                    const int t = t0 + u;
+                    int b = bin[0], o = t, ic = imgc[0];
+#pragma unroll
+                    for (int q = 1; q < 9; ++q)
+                        if (t >= beg[q]) { b = bin[q]; o = t - beg[q]; ic = imgc[q]; }
+                    im[u] = ic;
+                    jj[u] = t < tot ? B.list[(size_t)b * B.cap + o] : i;
+                }
+#pragma unroll
+                for (int u = 0; u < kVlBatch; ++u) {
+                    const int j = jj[u];
+                    ej[u] = P.eid[j]; uj[u] = P.up[j]; fj[u] = P.flags[j];
+                    kj[u] = P.kind[j]; cj[u] = P.cell[j];
+                    xj[u] = P.x[j]; yj[u] = P.y[j]; rj[u] = P.r[j];
+                }
+#pragma unroll
+                for (int u = 0; u < kVlBatch; ++u) {
+                    if (t0 + u >= tot) break;
+                    const int j = jj[u];
+                    if (eidi >= ej[u]) continue;
+                    if (K.world == WORLD_PERIODIC && (fj[u] & SKIP_CONTACT_DETECTION)) continue;
+                    if (upi == uj[u]) continue;
+                    // the reference's own candidate lists (its exit checks)
+                    const int cl = cell_list_of(K, ci, cj[u]);
+                    if (cl >= 0) {
+                        ncand[cl]++;
+                        if (ki != KIND_CLUMP && kj[u] != ki) raise_err(err, ERR_UNSUPPORTED);
+                    }
+                    if (ki == KIND_CLUMP || kj[u] != ki) continue;
+                    double ox, oy, ovx;
+                    image_offset(K, im[u], xi, yi, xj[u], yj[u], ox, oy, ovx);
+                    const double qx = xi - (xj[u] + ox), qy = yi - (yj[u] + oy);
+                    if ((ri + rj[u]) > sqrt(qx * qx + qy * qy)) {
+                        if (nl < kMaxList) s_list[nl][tid] = j;
+                        ++nl;
                     }
                 }
             }
@@ -620,27 +632,60 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
             ncand_total = (unsigned long long)(ncand[0] + ncand[1] + ncand[2]);
         }
         const int nlisted = min(nl, kMaxList);
+        // the pairs of the previous contact list, links and partners loaded
+        // a group at a time
         const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
-        for (int e = os; e < os + oc; ++e) {
-            const int c = old_t.link[e];
-            if (c < 0) continue;
-            const int j = old_t.cj[c];
-            bool listed = false;
-            for (int k = 0; k < nlisted; ++k) listed |= s_list[k][tid] == j;
-            if (listed) continue;
-            if (nl < kMaxList) s_list[nl][tid] = j | kExtraBit;
-            ++nl;
+        for (int e0 = os; e0 < os + oc; e0 += kVlBatch) {
+            int cc[kVlBatch], jo[kVlBatch];
+#pragma unroll
+            for (int u = 0; u < kVlBatch; ++u) cc[u] = e0 + u < os + oc ? old_t.link[e0 + u] : -1;
+#pragma unroll
+            for (int u = 0; u < kVlBatch; ++u) jo[u] = cc[u] >= 0 ? old_t.cj[cc[u]] : -1;
+#pragma unroll
+            for (int u = 0; u < kVlBatch; ++u) {
+                if (cc[u] < 0) continue;
+                const int j = jo[u];
+                bool listed = false;
+                for (int k = 0; k < nlisted; ++k) listed |= s_list[k][tid] == j;
+                if (listed) continue;
+                if (nl < kMaxList) s_list[nl][tid] = j | kExtraBit;
+                ++nl;
+            }
         }
         if (nl > kMaxList) { raise_err(err, ERR_ROW_OVERFLOW); nl = kMaxList; }
-        for (int a = 1; a < nl; ++a) {          // ascending partner ID
-            const int v = s_list[a][tid];
-            const int kv = P.eid[v & ~kExtraBit];
-            int b = a - 1;
-            while (b >= 0 && P.eid[s_list[b][tid] & ~kExtraBit] > kv) {
-                s_list[b + 1][tid] = s_list[b][tid];
-                --b;
+        // ascending partner ID: up to eight entries through a sorting network
+        // on registers (keys loaded together), longer rows by insertion
+        if (nl <= 8) {
+            int v[8], key[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = k < nl ? s_list[k][tid] : 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) key[k] = k < nl ? P.eid[v[k] & ~kExtraBit] : INT_MAX;
+            constexpr int kNet[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
+                                         {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
+                                         {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+            for (int c = 0; c < 19; ++c) {
+                const int a = kNet[c][0], b = kNet[c][1];
+                const bool sw = key[a] > key[b];
+                const int ka = sw ? key[b] : key[a], kb = sw ? key[a] : key[b];
+                const int va = sw ? v[b] : v[a], vb = sw ? v[a] : v[b];
+                key[a] = ka; key[b] = kb; v[a] = va; v[b] = vb;
             }
-            s_list[b + 1][tid] = v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (k < nl) s_list[k][tid] = v[k];
+        } else {
+            for (int a = 1; a < nl; ++a) {
+                const int v = s_list[a][tid];
+                const int kv = P.eid[v & ~kExtraBit];
+                int b = a - 1;
+                while (b >= 0 && P.eid[s_list[b][tid] & ~kExtraBit] > kv) {
+                    s_list[b + 1][tid] = s_list[b][tid];
+                    --b;
+                }
+                s_list[b + 1][tid] = v;
+            }
         }
     }
     count_add(&counter_slot(cnt)->cand, ncand_total);

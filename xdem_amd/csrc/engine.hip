@@ -534,6 +534,25 @@ constexpr int kExtraBit = 1 << 30;
 constexpr int kMaxList = 48;       // listed pairs j>i per particle
 constexpr int kVlBatch = 4;        // candidates loaded together
 
+// Up to eight (key, value) pairs held in registers, sorted by ascending key
+// with Batcher's odd-even merge network (19 compare-exchanges; it sorts
+// every 0-1 sequence, hence every sequence).  Unused entries carry key
+// INT_MAX and end up last.
+template <class V>
+__device__ __forceinline__ void sort8(int (&key)[8], V (&v)[8]) {
+    constexpr int kNet[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
+                                 {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
+                                 {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+    for (int c = 0; c < 19; ++c) {
+        const int a = kNet[c][0], b = kNet[c][1];
+        const bool sw = key[a] > key[b];
+        const int ka = sw ? key[b] : key[a], kb = sw ? key[a] : key[b];
+        const V va = sw ? v[b] : v[a], vb = sw ? v[a] : v[b];
+        key[a] = ka; key[b] = kb; v[a] = va; v[b] = vb;
+    }
+}
+
 template <bool ONE>
 __global__ void __launch_bounds__(kBlock)
 k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
@@ -661,17 +680,7 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
             for (int k = 0; k < 8; ++k) v[k] = k < nl ? s_list[k][tid] : 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) key[k] = k < nl ? P.eid[v[k] & ~kExtraBit] : INT_MAX;
-            constexpr int kNet[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
-                                         {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
-                                         {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
-#pragma unroll
-            for (int c = 0; c < 19; ++c) {
-                const int a = kNet[c][0], b = kNet[c][1];
-                const bool sw = key[a] > key[b];
-                const int ka = sw ? key[b] : key[a], kb = sw ? key[a] : key[b];
-                const int va = sw ? v[b] : v[a], vb = sw ? v[a] : v[b];
-                key[a] = ka; key[b] = kb; v[a] = va; v[b] = vb;
-            }
+            sort8(key, v);
 #pragma unroll
             for (int k = 0; k < 8; ++k)
                 if (k < nl) s_list[k][tid] = v[k];
@@ -736,6 +745,19 @@ k_in_sort(Parts P, const int* __restrict__ rebuild, uint32_t* err) {
     const size_t n = (size_t)P.n;
     int2* refs = P.in_ref + p;                   // entry k at refs[k * n]
     const int nin = min(P.in_cnt[p], kInCap);
+    if (nin <= 8) {                  // loaded together, sorted on registers
+        int key[8];
+        int2 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = k < nin ? refs[k * n] : make_int2(INT_MAX, 0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) key[k] = v[k].x;
+        sort8(key, v);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nin) refs[k * n] = v[k];
+        return;
+    }
     for (int a = 1; a < nin; ++a) {
         const int2 v = refs[a * n];
         int b = a - 1;

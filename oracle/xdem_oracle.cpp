@@ -626,7 +626,9 @@ static int step_once(World& W) {
     for (int i = 0; i < n; ++i) {
         int ix = (int)std::floor((W.x[i] - W.xmin) / lx);
         int iy = (int)std::floor((W.y[i] - W.ymin) / ly);
-        ix = std::min(std::max(ix, 0), nx - 1);   // x == xmax: see DESIGN
+        // x == xmax (the wrap is strict) gives index nx, which the reference
+        // writes into the next row (undefined); clamped to the last cell
+        ix = std::min(std::max(ix, 0), nx - 1);
         iy = std::min(std::max(iy, 0), ny - 1);
         cx[i] = ix; cy[i] = iy;
         bins[(size_t)(iy + 1) * nx + ix].push_back(i);

@@ -385,3 +385,21 @@ def test_nonfinite_geometry_repair_among_other_pairs(precision):
     if precision == 0:
         assert_parity(gpu, ref, env, "repaired pairs among others")
     eng.close()
+
+
+@pytest.mark.parametrize("name", ["pb_ellipse", "le_poly", "pb_circle"])
+def test_particles_on_the_upper_box_edge(name):
+    """x == xmax and y == ymax survive the strict wrap
+    (src/PeriodicBoundaryWorld.cpp:129-142) and give the cell index n, which
+    the reference writes into the next row (undefined).  Engine and oracle
+    bin such particles in the last cell and agree to the parity bars; for
+    these boxes floor((xmax - xmin) / len) is n exactly, so the case is
+    reached."""
+    s = helpers.state(name)
+    c = s.cfg
+    s.x[0] = c.xmax
+    s.y[1] = c.ymax
+    s.x[2], s.y[2] = c.xmax, c.ymax
+    print(f"[gpu] particles on the upper box edge {name}", flush=True)
+    ref, env = oracle_with_envelope(s)
+    assert_parity(run_gpu(s), ref, env, f"{name} on the box edge")

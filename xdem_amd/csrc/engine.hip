@@ -414,6 +414,9 @@ k_bin(Parts P, Frame F, const Pack* __restrict__ packs, Bins B, Table old_t, Tab
     if (rb) P.in_cnt[p] = 0;
     int ix = cell_index<PREC>(x, K.xmin, K.lenx);
     int iy = cell_index<PREC>(y, K.ymin, K.leny);
+    // x == xmax (the wrap above is strict, src/PeriodicBoundaryWorld.cpp:129-142)
+    // gives index nx, which the reference writes into the next row
+    // (undefined); clamped to the last cell, as the oracle does
     ix = min(max(ix, 0), K.nx - 1);
     iy = min(max(iy, 0), K.ny - 1);
     int grc = -1;

@@ -403,3 +403,33 @@ def test_particles_on_the_upper_box_edge(name):
     print(f"[gpu] particles on the upper box edge {name}", flush=True)
     ref, env = oracle_with_envelope(s)
     assert_parity(run_gpu(s), ref, env, f"{name} on the box edge")
+
+
+@pytest.mark.parametrize("d", [1.19, 1.1, 0.9])
+def test_two_circle_stress_closed_form_engine(d):
+    """The engine's stress of two resting overlapping circles against the
+    closed form kn delta d / (Lx Ly) (sign and branch orientation of
+    src/BaseParticle.cpp:85, 148)."""
+    from tests.test_oracle_math import two_circle_stress, two_circles
+    print(f"[gpu] two-circle stress d={d}", flush=True)
+    eng = _engine(two_circles(d))
+    eng.step(1)
+    st, ref = eng.stress()[0], two_circle_stress(d)
+    assert st[0] > 0 and abs(st[0] - ref[0]) <= 1e-12 * ref[0]
+    assert np.all(st[1:] == 0.0)
+    eng.close()
+
+
+@pytest.mark.parametrize("name", ["pb_ellipse", "le_poly", "pb_circle"])
+def test_forces_balance_engine(name):
+    """The engine's particle forces sum to zero up to rounding (every contact
+    adds F to P1 and -F to P2)."""
+    print(f"[gpu] force balance {name}", flush=True)
+    eng = _engine(helpers.state(name))
+    eng.step(3)
+    f, c = eng.forces(), eng.contacts()
+    scale = float(np.sum(np.hypot(c["fn_x"] + c["fs_x"], c["fn_y"] + c["fs_y"])))
+    assert scale > 0
+    for k in ("fx", "fy"):
+        assert abs(float(np.sum(f[k]))) <= 1e-12 * scale, k
+    eng.close()

@@ -375,3 +375,54 @@ def test_pw_half_separation_implies_no_oracle_contact():
     assert culled_total > 1000
     # the test is worth running: most pairs apart are caught
     assert culled_uniform > 0.7 * noncontact_uniform
+
+
+def two_circles(d, r1=0.5, r2=0.7, L=20.0, kn=1e4):
+    """Two resting circles on the x axis, centres d apart (d < r1 + r2)."""
+    from xdem_amd.state import KIND_CIRCLE, WORLD_PERIODIC, State, WorldConfig, particle_properties
+    kind = np.full(2, KIND_CIRCLE, np.int32)
+    r = np.array([r1, r2])
+    m, inertia, _ = particle_properties(kind, r, np.ones(2))
+    cfg = WorldConfig(world_kind=WORLD_PERIODIC, xmin=-L / 2, xmax=L / 2, ymin=-L / 2,
+                      ymax=L / 2, dt=1e-4, kn=kn)
+    z = np.zeros(2)
+    return State(cfg, kind, np.arange(2, dtype=np.int32), np.zeros(2, np.int32), r, np.ones(2),
+                 m, inertia, np.array([-0.25, -0.25 + d]), z.copy(), z.copy(), z.copy(),
+                 z.copy(), z.copy(), z.copy(), z.copy())
+
+
+def two_circle_stress(d, r1=0.5, r2=0.7, L=20.0, kn=1e4):
+    """Closed form of the stress after one step: normal n = (1, 0) from P1 to
+    P2, force on P1 F = (-kn delta, 0) (no relative velocity, so no damping
+    and no shear), branch x1 - x2 = (-d, 0); stress = F (x) branch / (Lx Ly)
+    (src/BaseParticle.cpp:85, 148; src/BaseWorld.cpp:158-176)."""
+    delta = (r1 + r2) - d
+    return np.array([kn * delta * d / (L * L), 0.0, 0.0, 0.0])
+
+
+@pytest.mark.parametrize("d", [1.19, 1.1, 0.9])
+def test_two_circle_stress_closed_form(d):
+    w = po.OracleWorld(two_circles(d))
+    w.step(1)
+    ref = two_circle_stress(d)
+    st = w.stress()
+    assert st[0] > 0 and abs(st[0] - ref[0]) <= 1e-12 * ref[0]
+    assert np.all(st[1:] == 0.0)
+    c = w.contacts()
+    assert list(c["i"]) == [0] and list(c["j"]) == [1]
+    assert abs(c["fn_x"][0] + 1e4 * ((0.5 + 0.7) - d)) <= 1e-12 * 1e4 * ((0.5 + 0.7) - d)
+
+
+@pytest.mark.parametrize("name", ["pb_ellipse", "le_poly", "pb_circle"])
+def test_forces_balance(name):
+    """Every contact adds F to P1 and -F to P2, so the particle forces sum to
+    zero up to rounding; a dropped or flipped reaction would leave a sum of
+    the order of the forces themselves."""
+    w = po.OracleWorld(helpers.state(name))
+    w.step(3)
+    f = w.forces()
+    c = w.contacts()
+    scale = float(np.sum(np.hypot(c["fn_x"] + c["fs_x"], c["fn_y"] + c["fs_y"])))
+    assert scale > 0
+    for k in ("fx", "fy"):
+        assert abs(float(np.sum(f[k]))) <= 1e-12 * scale, k

@@ -844,12 +844,11 @@ __device__ __forceinline__ bool pw_separated(T dx, T dy, double a1, double b1, d
 template <int PREC, bool ONE>
 __device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
                                             const Pack* __restrict__ packs, const Table& old_t,
-                                            const Table& new_t, bool rb, int e, uint32_t* err,
+                                            const Table& new_t, bool rb, int e, int i, int code,
+                                            int link_e, uint32_t* err,
                                             unsigned long long& n_coarse, int4& item,
                                             float (&feat)[kMlpIn]) {
     bool push = false;
-    const int i = new_t.pr_i[e];
-    const int code = new_t.pr_j[e];
     const int j = code & ~kExtraBit;
     if (i >= 0 && i < P.n && j >= 0 && j < P.n) {
         const Pack K = packs[ONE ? 0 : P.pack[i]];
@@ -866,7 +865,7 @@ __device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
         // list: the same slot when the list was not rebuilt
         int found = -1;
         if (!rb) {
-            found = old_t.link[e];
+            found = link_e;
         } else {
             const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
             for (int t = os; t < os + oc; ++t) {
@@ -950,6 +949,17 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
     xcd_chunk(s_pre[kSeg], wbeg, wend);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long n_coarse = 0;
+    // the next round's slot, pair and link are loaded while this round runs
+    auto slot_of = [&](int w) {
+        const int lo = segment_of(s_pre, w);
+        return lo * new_t.seg_cap + (w - s_pre[lo]);
+    };
+    int en = -1, in_ = 0, cn = 0, ln = -1;
+    if (wbeg + (int)threadIdx.x < wend) {
+        en = slot_of(wbeg + threadIdx.x);
+        in_ = new_t.pr_i[en]; cn = new_t.pr_j[en];
+        if (!rb) ln = old_t.link[en];
+    }
     for (int w0 = wbeg; w0 < wend; w0 += kBlock) {
         const int w = w0 + threadIdx.x;
         bool push = false;
@@ -957,12 +967,15 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
         float feat[kMlpIn];
         if (PREC == kNN)
             for (int k = 0; k < kMlpIn; ++k) feat[k] = 0.f;
-        if (w < wend) {
-            const int lo = segment_of(s_pre, w);
-            const int e = lo * new_t.seg_cap + (w - s_pre[lo]);
-            push = coarse_slot<PREC, ONE>(P, F, packs, old_t, new_t, rb, e, err, n_coarse, item,
-                                          feat);
+        const int e = en, ic = in_, cc = cn, lk = ln;
+        if (w + kBlock < wend) {
+            en = slot_of(w + kBlock);
+            in_ = new_t.pr_i[en]; cn = new_t.pr_j[en];
+            if (!rb) ln = old_t.link[en];
         }
+        if (w < wend)
+            push = coarse_slot<PREC, ONE>(P, F, packs, old_t, new_t, rb, e, ic, cc, lk, err,
+                                          n_coarse, item, feat);
         // block compaction: one counter update per block and round
         const unsigned long long m = __ballot(push ? 1 : 0);
         const int before = __popcll(m & ((1ULL << lane) - 1ULL));

@@ -867,10 +867,17 @@ __device__ __forceinline__ bool coarse_slot(const Parts& P, const Frame& F,
         if (!rb) {
             found = link_e;
         } else {
+            // links and partners of P1's previous row, a group at a time
             const int os = old_t.row_start[i], oc = old_t.row_cnt[i];
-            for (int t = os; t < os + oc; ++t) {
-                const int c = old_t.link[t];
-                if (c >= 0 && old_t.cj[c] == j) { found = c; break; }
+            for (int t0 = os; t0 < os + oc && found < 0; t0 += kVlBatch) {
+                int cc[kVlBatch], jo[kVlBatch];
+#pragma unroll
+                for (int u = 0; u < kVlBatch; ++u) cc[u] = t0 + u < os + oc ? old_t.link[t0 + u] : -1;
+#pragma unroll
+                for (int u = 0; u < kVlBatch; ++u) jo[u] = cc[u] >= 0 ? old_t.cj[cc[u]] : -1;
+#pragma unroll
+                for (int u = 0; u < kVlBatch; ++u)
+                    if (found < 0 && cc[u] >= 0 && jo[u] == j) found = cc[u];
             }
         }
         int lst;

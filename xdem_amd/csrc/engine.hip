@@ -1,7 +1,7 @@
 // engine.hip -- MI355X (gfx950) DEM step engine behind the C-ABI of
 // include/xdem.h.
 //
-// One step of every packing is five launches on the engine's stream:
+// One step of every packing is six launches on the engine's stream:
 //   k_bin         wrap positions (modifyParticlePosition), cell index, the
 //                 per-particle ellipse frame of this step; on a list-rebuild
 //                 step also fills fixed-capacity cell bins and the two ghost
@@ -10,8 +10,8 @@
 //                 reference's candidate filters; keeps pairs whose bounding
 //                 circles come within a skin; rows sorted by partner ID,
 //                 block-compacted pair list, per-particle incoming index
-//   k_coarse      (rebuild steps) incoming index in ID order; then one lane
-//                 per listed pair: the reference's candidate rule
+//   k_in_sort     (rebuild steps only) incoming index in ID order
+//   k_coarse      one lane per listed pair: the reference's candidate rule
 //                 and image offset from this step's cells, bounding-circle
 //                 test; erases / carries history; compacts the survivors
 //   k_contact     one lane per surviving pair: ZPM / circle geometry,
@@ -740,9 +740,11 @@ k_vl_build(Parts P, Pack* __restrict__ packs, Bins B, Table old_t, Table new_t,
     }
 }
 
-// Rebuild steps: order particle p's incoming index by the ID of P1 (run by
-// the coarse pass, one thread per column, before the force sums read it).
-__device__ __forceinline__ void sort_incoming(const Parts& P, int p) {
+// Rebuild steps: order each particle's incoming index by the ID of P1.
+__global__ void __launch_bounds__(kBlock)
+k_in_sort(Parts P, const int* __restrict__ rebuild, uint32_t* err) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (*rebuild == 0 || p >= P.n || *err) return;
     const size_t n = (size_t)P.n;
     int2* refs = P.in_ref + p;                   // entry k at refs[k * n]
     const int nin = min(P.in_cnt[p], kInCap);
@@ -950,11 +952,6 @@ k_coarse(Parts P, Frame F, const Pack* __restrict__ packs, Table old_t, Table ne
     }
     __syncthreads();
     if (!s_go) return;
-    // rebuild steps: the incoming index in ID order (written by the list
-    // build; read only by the integrator, after this kernel)
-    if (rb)
-        for (int p = xcd_block() * kBlock + threadIdx.x; p < P.n; p += gridDim.x * kBlock)
-            sort_incoming(P, p);
     int wbeg, wend;
     xcd_chunk(s_pre[kSeg], wbeg, wend);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -2741,6 +2738,7 @@ int launch_step_t(xdem_engine* E) {
                        E->packs_d, E->B, old_t, new_t, E->W, rb, rb_next, E->err_d);
     hipLaunchKernelGGL(k_vl_build<ONE>, dim3(gn), dim3(kBlock), 0, E->stream, E->P, E->packs_d,
                        E->B, old_t, new_t, rb, E->err_d, E->cnt_d);
+    hipLaunchKernelGGL(k_in_sort, dim3(gn), dim3(kBlock), 0, E->stream, E->P, rb, E->err_d);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (E->timing) {
         while (E->ev_pool.size() < E->ev_used + 2) {
